@@ -1,0 +1,359 @@
+// C ABI: the cuFINUFFT guru and simple interfaces (include/cufinufft.h) and
+// the extensions of include/finufft_amd.h, over fa::Plan<T>.
+//
+// Argument checks and their order follow the reference C layer
+// (src/cuda/c_interface.cpp:14-217): dim, then the mode array, then the plan
+// constructor's own checks; every exception becomes an integer code.
+#include <cufinufft.h>
+#include <finufft_amd.h>
+
+#include <algorithm>
+#include <array>
+#include <vector>
+#include <cstdint>
+#include <limits>
+#include <memory>
+
+#include "kernel_math.hpp"
+#include "plan.hpp"
+
+using fa::Plan;
+using fa::safe_call;
+
+namespace {
+
+constexpr int64_t I32MAX = std::numeric_limits<int32_t>::max();
+
+// Mode sizes must be in (0, INT32_MAX] with product <= INT32_MAX; type 3
+// ignores them (c_interface.cpp:14-31).
+bool invalid_modes(int type, int dim, const int64_t *m64, int m32[3]) {
+  if (type == 3) {
+    m32[0] = m32[1] = m32[2] = 1;
+    return false;
+  }
+  int64_t tot = 1;
+  for (int i = 0; i < dim; ++i) {
+    if (m64[i] > I32MAX || m64[i] <= 0) return true;
+    m32[i] = int(m64[i]);
+    tot *= m64[i];
+  }
+  for (int i = dim; i < 3; ++i) m32[i] = 1;
+  return tot > I32MAX;
+}
+
+template <typename T>
+Plan<T> *as_plan(void *p) {
+  if (!p) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null plan");
+  return static_cast<Plan<T> *>(p);
+}
+
+template <typename T, typename P>
+int makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntr, T eps, P *out,
+             const cufinufft_opts *opts) {
+  return safe_call([&] {
+    if (dim < 1 || dim > 3) fa::fail(FINUFFT_ERR_DIM_NOTVALID, "invalid dim");
+    int m32[3];
+    if (invalid_modes(type, dim, n_modes, m32))
+      fa::fail(FINUFFT_ERR_NDATA_NOTVALID, "invalid mode sizes");
+    cufinufft_opts o;
+    if (opts)
+      o = *opts;
+    else
+      cufinufft_default_opts(&o);
+    if (!out) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "null plan pointer");
+    *out = reinterpret_cast<P>(new Plan<T>(type, dim, m32, iflag, ntr, eps, o));
+  });
+}
+
+template <typename T>
+int setpts(void *plan, int64_t M, const T *x, const T *y, const T *z, int N, const T *s,
+           const T *t, const T *u) {
+  return safe_call([&] {
+    if (M > I32MAX) fa::fail(FINUFFT_ERR_NDATA_NOTVALID, "M exceeds INT32_MAX");
+    as_plan<T>(plan)->setpts(int(M), x, y, z, N, s, t, u);
+  });
+}
+
+template <typename T>
+int simple(int dim, int type, int ntr, int64_t nj, const std::array<const T *, 3> &xyz,
+           fa::cplx_t<T> *cj, int iflag, T eps, const std::array<int64_t, 3> &modes, int64_t nk,
+           const std::array<const T *, 3> &stu, fa::cplx_t<T> *fk, const cufinufft_opts *opts) {
+  return safe_call([&] {
+    if (dim < 1 || dim > 3) fa::fail(FINUFFT_ERR_DIM_NOTVALID, "invalid dim");
+    if (nj > I32MAX || nk > I32MAX) fa::fail(FINUFFT_ERR_NDATA_NOTVALID, "size exceeds INT32_MAX");
+    int m32[3];
+    if (invalid_modes(type, dim, modes.data(), m32))
+      fa::fail(FINUFFT_ERR_NDATA_NOTVALID, "invalid mode sizes");
+    cufinufft_opts o;
+    if (opts)
+      o = *opts;
+    else
+      cufinufft_default_opts(&o);
+    auto plan = std::make_unique<Plan<T>>(type, dim, m32, iflag, ntr, eps, o);
+    plan->setpts(int(nj), xyz[0], xyz[1], xyz[2], int(nk), stu[0], stu[1], stu[2]);
+    plan->execute(cj, fk);
+  });
+}
+
+template <typename T>
+fa::cplx_t<T> *nc(const fa::cplx_t<T> *p) {
+  return const_cast<fa::cplx_t<T> *>(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+void cufinufft_default_opts(cufinufft_opts *opts) {
+  if (!opts) return;
+  // Values of src/cuda/c_interface.cpp:153-172.
+  opts->modeord = 0;
+  opts->gpu_device_id = 0;
+  opts->gpu_spreadinterponly = 0;
+  opts->gpu_method = 0;
+  opts->gpu_sort = 1;
+  opts->gpu_kerevalmeth = 1;
+  opts->upsampfac = 0.0;
+  opts->gpu_maxsubprobsize = 1024;
+  opts->gpu_obinsizex = 0;
+  opts->gpu_obinsizey = 0;
+  opts->gpu_obinsizez = 0;
+  opts->gpu_binsizex = 0;
+  opts->gpu_binsizey = 0;
+  opts->gpu_binsizez = 0;
+  opts->gpu_maxbatchsize = 0;
+  opts->gpu_np = 0;
+  opts->debug = 0;
+  opts->gpu_stream = nullptr;
+}
+
+int cufinufft_makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntr, double eps,
+                       cufinufft_plan *d_plan_ptr, const cufinufft_opts *opts) {
+  return makeplan<double>(type, dim, n_modes, iflag, ntr, eps, d_plan_ptr, opts);
+}
+int cufinufftf_makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntr, float eps,
+                        cufinufftf_plan *d_plan_ptr, const cufinufft_opts *opts) {
+  return makeplan<float>(type, dim, n_modes, iflag, ntr, eps, d_plan_ptr, opts);
+}
+
+int cufinufft_setpts(cufinufft_plan p, int64_t M, const double *x, const double *y,
+                     const double *z, int N, const double *s, const double *t, const double *u) {
+  return setpts<double>(p, M, x, y, z, N, s, t, u);
+}
+int cufinufftf_setpts(cufinufftf_plan p, int64_t M, const float *x, const float *y,
+                      const float *z, int N, const float *s, const float *t, const float *u) {
+  return setpts<float>(p, M, x, y, z, N, s, t, u);
+}
+
+int cufinufft_execute(cufinufft_plan p, hipDoubleComplex *c, hipDoubleComplex *fk) {
+  return safe_call([&] { as_plan<double>(p)->execute(c, fk); });
+}
+int cufinufftf_execute(cufinufftf_plan p, hipFloatComplex *c, hipFloatComplex *fk) {
+  return safe_call([&] { as_plan<float>(p)->execute(c, fk); });
+}
+
+int cufinufft_destroy(cufinufft_plan p) {
+  return safe_call([&] { delete as_plan<double>(p); });
+}
+int cufinufftf_destroy(cufinufftf_plan p) {
+  return safe_call([&] { delete as_plan<float>(p); });
+}
+
+int cufinufft_execute_adjoint(cufinufft_plan p, hipDoubleComplex *c, hipDoubleComplex *fk) {
+  return safe_call([&] { as_plan<double>(p)->execute_adjoint(c, fk); });
+}
+int cufinufftf_execute_adjoint(cufinufftf_plan p, hipFloatComplex *c, hipFloatComplex *fk) {
+  return safe_call([&] { as_plan<float>(p)->execute_adjoint(c, fk); });
+}
+
+int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout, double *dout) {
+  return safe_call([&] {
+    auto fill = [&](const auto *p) {
+      if (iout) {
+        const int64_t v[16] = {p->type(),       p->dim(),       p->ns(),     p->nf(0),
+                               p->nf(1),        p->nf(2),       p->method(), p->binsize(0),
+                               p->binsize(1),   p->binsize(2),  p->batchsize(), p->M(),
+                               p->nbins_total(), 0,             0,           0};
+        for (int i = 0; i < 16; ++i) iout[i] = v[i];
+      }
+      if (dout) {
+        dout[0] = p->beta();
+        dout[1] = p->upsampfac();
+        dout[2] = dout[3] = 0;
+      }
+    };
+    void *q = const_cast<void *>(plan);
+    if (is_double)
+      fill(as_plan<double>(q));
+    else
+      fill(as_plan<float>(q));
+  });
+}
+
+int finufft_amd_copy_sort(const void *plan, int is_double, int *perm, int *binstart) {
+  return safe_call([&] {
+    auto copy = [&](const auto *p) {
+      const hipStream_t s = p->stream();
+      if (perm && p->M() > 0)
+        fa::hip_check(hipMemcpyAsync(perm, p->sort_perm(), size_t(p->M()) * sizeof(int),
+                                     hipMemcpyDeviceToDevice, s),
+                      "hipMemcpyAsync(perm)");
+      if (binstart && p->bin_start())
+        fa::hip_check(hipMemcpyAsync(binstart, p->bin_start(),
+                                     (size_t(p->nbins_total()) + 1) * sizeof(int),
+                                     hipMemcpyDeviceToDevice, s),
+                      "hipMemcpyAsync(binstart)");
+      fa::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(copy_sort)");
+    };
+    void *q = const_cast<void *>(plan);
+    if (is_double)
+      copy(as_plan<double>(q));
+    else
+      copy(as_plan<float>(q));
+  });
+}
+
+int64_t finufft_amd_next235(int64_t n, int64_t f) { return fa::next235(n, f); }
+
+void finufft_amd_gaussquad(int n, double *x, double *w) {
+  if (n > 0 && x && w) fa::gaussquad(n, x, w);
+}
+
+int finufft_amd_kernel_shape(double tol, int is_double, double upsampfac, int kerevalmeth,
+                             int *ns, double *beta) {
+  return safe_call([&] {
+    if (upsampfac != 2.0 && upsampfac != 1.25) {
+      if (kerevalmeth == 1) fa::fail(FINUFFT_ERR_HORNER_WRONG_BETA, "Horner needs sigma 2 or 1.25");
+      if (upsampfac <= 1.0) fa::fail(FINUFFT_ERR_UPSAMPFAC_TOO_SMALL, "upsampfac must exceed 1");
+    }
+    fa::KernelShape ks;
+    if (is_double) {
+      const double e = std::max(tol, std::numeric_limits<double>::epsilon());
+      ks = fa::kernel_shape(e, e / 10.0, upsampfac, fa::MAX_NS);
+    } else {
+      const float e = std::max(float(tol), std::numeric_limits<float>::epsilon());
+      ks = fa::kernel_shape(double(e), double(e / 10.0f), upsampfac, fa::MAX_NS_F32);
+    }
+    if (ns) *ns = ks.ns;
+    if (beta) *beta = ks.beta;
+  });
+}
+
+int finufft_amd_horner_coeffs(int ns, double upsampfac, double *out, int nc_max) {
+  if (ns < 2 || ns > fa::MAX_NS || !out) return 0;
+  const int nc = std::min(nc_max, fa::horner_nc_sigma2(ns));
+  const double bo = upsampfac == 2.0 ? (ns == 2 ? 2.20 : ns == 3 ? 2.26 : ns == 4 ? 2.38 : 2.30)
+                                     : 0.97 * fa::PI * (1.0 - 1.0 / (2.0 * upsampfac));
+  const std::vector<double> c = fa::horner_coeffs(ns, upsampfac, bo * ns, nc);
+  std::copy(c.begin(), c.end(), out);
+  return nc;
+}
+
+void finufft_amd_fold_rescale_f(const float *x, int64_t m, int n, float *out) {
+  for (int64_t i = 0; i < m; ++i) out[i] = fa::fold_rescale(x[i], n);
+}
+void finufft_amd_fold_rescale_d(const double *x, int64_t m, int n, double *out) {
+  for (int64_t i = 0; i < m; ++i) out[i] = fa::fold_rescale(x[i], n);
+}
+
+int64_t finufft_amd_fine_grid_len(int64_t m, double upsampfac, int ns) {
+  return fa::fine_grid_len(m, upsampfac, ns);
+}
+
+// ---- simple interfaces (c_interface.cpp:219-470)
+
+#define FA_SIMPLE_1D(P, T, CT)                                                                 \
+  int P##1d1many(int nt, int64_t nj, const T *x, const CT *c, int fl, T e, int64_t ms, CT *fk, \
+                 const cufinufft_opts *o) {                                                    \
+    return simple<T>(1, 1, nt, nj, {x, nullptr, nullptr}, nc<T>(c), fl, e, {ms, 1, 1}, 0, {},  \
+                     fk, o);                                                                   \
+  }                                                                                            \
+  int P##1d1(int64_t nj, const T *x, const CT *c, int fl, T e, int64_t ms, CT *fk,             \
+             const cufinufft_opts *o) {                                                        \
+    return P##1d1many(1, nj, x, c, fl, e, ms, fk, o);                                          \
+  }                                                                                            \
+  int P##1d2many(int nt, int64_t nj, const T *x, CT *c, int fl, T e, int64_t ms, const CT *fk, \
+                 const cufinufft_opts *o) {                                                    \
+    return simple<T>(1, 2, nt, nj, {x, nullptr, nullptr}, c, fl, e, {ms, 1, 1}, 0, {},         \
+                     nc<T>(fk), o);                                                            \
+  }                                                                                            \
+  int P##1d2(int64_t nj, const T *x, CT *c, int fl, T e, int64_t ms, const CT *fk,             \
+             const cufinufft_opts *o) {                                                        \
+    return P##1d2many(1, nj, x, c, fl, e, ms, fk, o);                                          \
+  }                                                                                            \
+  int P##1d3many(int nt, int64_t nj, const T *x, const CT *c, int fl, T e, int64_t nk,         \
+                 const T *s, CT *fk, const cufinufft_opts *o) {                                \
+    return simple<T>(1, 3, nt, nj, {x, nullptr, nullptr}, nc<T>(c), fl, e, {0, 0, 0}, nk,      \
+                     {s, nullptr, nullptr}, fk, o);                                            \
+  }                                                                                            \
+  int P##1d3(int64_t nj, const T *x, const CT *c, int fl, T e, int64_t nk, const T *s, CT *fk, \
+             const cufinufft_opts *o) {                                                        \
+    return P##1d3many(1, nj, x, c, fl, e, nk, s, fk, o);                                       \
+  }
+
+#define FA_SIMPLE_2D(P, T, CT)                                                                 \
+  int P##2d1many(int nt, int64_t nj, const T *x, const T *y, const CT *c, int fl, T e,         \
+                 int64_t ms, int64_t mt, CT *fk, const cufinufft_opts *o) {                    \
+    return simple<T>(2, 1, nt, nj, {x, y, nullptr}, nc<T>(c), fl, e, {ms, mt, 1}, 0, {}, fk,   \
+                     o);                                                                       \
+  }                                                                                            \
+  int P##2d1(int64_t nj, const T *x, const T *y, const CT *c, int fl, T e, int64_t ms,         \
+             int64_t mt, CT *fk, const cufinufft_opts *o) {                                    \
+    return P##2d1many(1, nj, x, y, c, fl, e, ms, mt, fk, o);                                   \
+  }                                                                                            \
+  int P##2d2many(int nt, int64_t nj, const T *x, const T *y, CT *c, int fl, T e, int64_t ms,   \
+                 int64_t mt, const CT *fk, const cufinufft_opts *o) {                          \
+    return simple<T>(2, 2, nt, nj, {x, y, nullptr}, c, fl, e, {ms, mt, 1}, 0, {}, nc<T>(fk),   \
+                     o);                                                                       \
+  }                                                                                            \
+  int P##2d2(int64_t nj, const T *x, const T *y, CT *c, int fl, T e, int64_t ms, int64_t mt,   \
+             const CT *fk, const cufinufft_opts *o) {                                          \
+    return P##2d2many(1, nj, x, y, c, fl, e, ms, mt, fk, o);                                   \
+  }                                                                                            \
+  int P##2d3many(int nt, int64_t nj, const T *x, const T *y, const CT *c, int fl, T e,         \
+                 int64_t nk, const T *s, const T *t, CT *fk, const cufinufft_opts *o) {        \
+    return simple<T>(2, 3, nt, nj, {x, y, nullptr}, nc<T>(c), fl, e, {0, 0, 0}, nk,            \
+                     {s, t, nullptr}, fk, o);                                                  \
+  }                                                                                            \
+  int P##2d3(int64_t nj, const T *x, const T *y, const CT *c, int fl, T e, int64_t nk,         \
+             const T *s, const T *t, CT *fk, const cufinufft_opts *o) {                        \
+    return P##2d3many(1, nj, x, y, c, fl, e, nk, s, t, fk, o);                                 \
+  }
+
+#define FA_SIMPLE_3D(P, T, CT)                                                                 \
+  int P##3d1many(int nt, int64_t nj, const T *x, const T *y, const T *z, const CT *c, int fl,  \
+                 T e, int64_t ms, int64_t mt, int64_t mu, CT *fk, const cufinufft_opts *o) {   \
+    return simple<T>(3, 1, nt, nj, {x, y, z}, nc<T>(c), fl, e, {ms, mt, mu}, 0, {}, fk, o);    \
+  }                                                                                            \
+  int P##3d1(int64_t nj, const T *x, const T *y, const T *z, const CT *c, int fl, T e,         \
+             int64_t ms, int64_t mt, int64_t mu, CT *fk, const cufinufft_opts *o) {            \
+    return P##3d1many(1, nj, x, y, z, c, fl, e, ms, mt, mu, fk, o);                            \
+  }                                                                                            \
+  int P##3d2many(int nt, int64_t nj, const T *x, const T *y, const T *z, CT *c, int fl, T e,   \
+                 int64_t ms, int64_t mt, int64_t mu, const CT *fk, const cufinufft_opts *o) {  \
+    return simple<T>(3, 2, nt, nj, {x, y, z}, c, fl, e, {ms, mt, mu}, 0, {}, nc<T>(fk), o);    \
+  }                                                                                            \
+  int P##3d2(int64_t nj, const T *x, const T *y, const T *z, CT *c, int fl, T e, int64_t ms,   \
+             int64_t mt, int64_t mu, const CT *fk, const cufinufft_opts *o) {                  \
+    return P##3d2many(1, nj, x, y, z, c, fl, e, ms, mt, mu, fk, o);                            \
+  }                                                                                            \
+  int P##3d3many(int nt, int64_t nj, const T *x, const T *y, const T *z, const CT *c, int fl,  \
+                 T e, int64_t nk, const T *s, const T *t, const T *u, CT *fk,                  \
+                 const cufinufft_opts *o) {                                                    \
+    return simple<T>(3, 3, nt, nj, {x, y, z}, nc<T>(c), fl, e, {0, 0, 0}, nk, {s, t, u}, fk,   \
+                     o);                                                                       \
+  }                                                                                            \
+  int P##3d3(int64_t nj, const T *x, const T *y, const T *z, const CT *c, int fl, T e,         \
+             int64_t nk, const T *s, const T *t, const T *u, CT *fk,                           \
+             const cufinufft_opts *o) {                                                        \
+    return P##3d3many(1, nj, x, y, z, c, fl, e, nk, s, t, u, fk, o);                           \
+  }
+
+FA_SIMPLE_1D(cufinufft, double, hipDoubleComplex)
+FA_SIMPLE_1D(cufinufftf, float, hipFloatComplex)
+FA_SIMPLE_2D(cufinufft, double, hipDoubleComplex)
+FA_SIMPLE_2D(cufinufftf, float, hipFloatComplex)
+FA_SIMPLE_3D(cufinufft, double, hipDoubleComplex)
+FA_SIMPLE_3D(cufinufftf, float, hipFloatComplex)
+
+}  // extern "C"

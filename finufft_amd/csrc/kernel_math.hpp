@@ -1,0 +1,144 @@
+// Host+device math of the spreading kernel and the coordinate fold.
+//
+//  * fold_rescale: x in R -> [0, nf) with the reference GPU's rounding
+//    (fma round-to-nearest, then subtract and multiply rounded DOWN;
+//    include/cufinufft/spreadinterp.hpp:61-89). The round-down operations are
+//    computed exactly from round-to-nearest results plus their error terms
+//    (TwoSum / fma), so the same code gives bit-identical results on host and
+//    device without touching the FP rounding mode.
+//  * ES kernel phi(z) = exp(beta (sqrt(1 - z^2) - 1)), evaluated directly
+//    (spreadinterp.hpp:91-111) or by per-interval Horner polynomials whose
+//    coefficients are generated at plan time (horner.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace fa {
+
+#define FA_HD __host__ __device__ inline
+
+// ----- exact directed rounding built from round-to-nearest arithmetic -----
+
+FA_HD float next_down(float s) {
+  if (s == 0.0f) return -1.40129846e-45f;  // -denorm_min
+  uint32_t b;
+  memcpy(&b, &s, 4);
+  b = (s > 0.0f) ? b - 1u : b + 1u;
+  float r;
+  memcpy(&r, &b, 4);
+  return r;
+}
+FA_HD double next_down(double s) {
+  if (s == 0.0) return -4.9406564584124654e-324;
+  uint64_t b;
+  memcpy(&b, &s, 8);
+  b = (s > 0.0) ? b - 1u : b + 1u;
+  double r;
+  memcpy(&r, &b, 8);
+  return r;
+}
+
+// a - b rounded toward -inf. TwoSum gives the exact error e of s = RN(a - b)
+// (a - b = s + e exactly); when e < 0 the exact value lies below s.
+template <typename T>
+FA_HD T sub_rd(T a, T b) {
+  const T nb = -b;
+  const T s = a + nb;
+  const T bb = s - a;
+  const T e = (a - (s - bb)) + (nb - bb);
+  return (e < T(0)) ? next_down(s) : s;
+}
+
+// a * b rounded toward -inf; fma(a, b, -p) is the exact product error.
+template <typename T>
+FA_HD T mul_rd(T a, T b) {
+  const T p = a * b;
+  const T e = std::fma(a, b, -p);
+  return (e < T(0)) ? next_down(p) : p;
+}
+
+// Folds x (any real; [-pi, pi) is the nominal box) periodically and rescales
+// to [0, n). Matches the reference device fold_rescale bit for bit.
+template <typename T>
+FA_HD T fold_rescale(T x, int n) {
+  const T x2pi = T(0.159154943091895345554011992339482617);
+  T r = std::fma(x, x2pi, T(0.5));
+  r = sub_rd(r, std::floor(r));
+  return mul_rd(r, static_cast<T>(n));
+}
+
+// Bin of a folded coordinate, clamped to [0, nbins) as the reference's
+// compute_bin_index (spreadinterp.hpp:214-229).
+template <typename T>
+FA_HD int bin_of(T rescaled, T inv_binsize, int nbins) {
+  int b = int(std::floor(rescaled * inv_binsize));
+  b = b >= nbins ? b - 1 : b;
+  return b < 0 ? 0 : b;
+}
+
+// ----- polynomial degree table -----
+
+// Number of Horner coefficients for width w at sigma = 2 (degree
+// w + 1 + (w<=7) - (w==2), devel/get_degree_and_beta.m:24-25) and at
+// sigma = 1.25 (degree ceil(0.7 w + 1.3), :31). The sigma = 2 count is the
+// larger, so device code is compiled for it and pads sigma = 1.25 with
+// leading zeros (an exact no-op in Horner's rule).
+constexpr int horner_nc_sigma2(int w) { return w + 2 + (w <= 7 ? 1 : 0) - (w == 2 ? 1 : 0); }
+inline int horner_nc_sigma125(int w) { return int(std::ceil(0.7 * w + 1.3)) + 1; }
+
+constexpr int MAX_NS = 16;                       // largest kernel width (fp64)
+constexpr int MAX_NS_F32 = 12;                   // largest kernel width (fp32)
+constexpr int MAX_NC = horner_nc_sigma2(MAX_NS);  // 18
+
+// Kernel evaluation parameters shared by all spread/interp kernels.
+template <typename T>
+struct KerParams {
+  T beta;         // ES shape parameter
+  T es_c;         // (2/ns)^2
+  int horner;     // 1: Horner polynomial, 0: direct exp(sqrt())
+  const T *coef;  // device: coef[k*ns + i], k = 0..NC(ns)-1, ascending powers
+};
+
+// Fills ker[i] = phi at the ns grid offsets for a point whose first grid node
+// sits at xl = s - x_rescaled in [-ns/2, -ns/2 + 1].
+template <typename T, int NS>
+__device__ __forceinline__ void eval_kernel(T *ker, T xl, const KerParams<T> &kp) {
+  if (kp.horner) {
+    constexpr int NC = horner_nc_sigma2(NS);
+    const T z = std::fma(T(2), xl, T(NS - 1));
+#pragma unroll
+    for (int i = 0; i < NS; ++i) ker[i] = kp.coef[(NC - 1) * NS + i];
+#pragma unroll
+    for (int k = NC - 2; k >= 0; --k) {
+#pragma unroll
+      for (int i = 0; i < NS; ++i) ker[i] = std::fma(ker[i], z, kp.coef[k * NS + i]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const T a = xl + T(i);
+      const T zsq = kp.es_c * a * a;
+      ker[i] = (zsq < T(1)) ? std::exp(kp.beta * (std::sqrt(T(1) - zsq) - T(1))) : T(0);
+    }
+  }
+}
+
+// Start node and kernel values for one coordinate: s = ceil(x - ns/2)
+// (spreadinterp.hpp:154-196).
+template <typename T, int NS>
+__device__ __forceinline__ int start_and_kernel(T x, int nf, T *ker, const KerParams<T> &kp) {
+  T r = fold_rescale(x, nf);
+  // A non-finite coordinate folds to NaN; pin it to the grid so that every
+  // index stays in bounds (its contribution is then meaningless, as it is in
+  // the reference, but harmless).
+  r = (r >= T(0) && r < T(nf)) ? r : T(0);
+  const int s = int(std::ceil(r - T(NS * 0.5)));
+  eval_kernel<T, NS>(ker, T(s) - r, kp);
+  return s;
+}
+
+}  // namespace fa

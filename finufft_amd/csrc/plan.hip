@@ -1,0 +1,453 @@
+// NUFFT plan implementation (see plan.hpp).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "plan_kernels.hpp"
+
+namespace fa {
+
+// ------------------------------------------------------------ dispatch glue
+
+template <typename T>
+void spread(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
+            hipStream_t stream) {
+  switch (ctx.dim) {
+    case 1: return spread_dim<T, 1>(ctx, c, fw, blk, stream);
+    case 2: return spread_dim<T, 2>(ctx, c, fw, blk, stream);
+    default: return spread_dim<T, 3>(ctx, c, fw, blk, stream);
+  }
+}
+
+template <typename T>
+void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
+            hipStream_t stream) {
+  switch (ctx.dim) {
+    case 1: return interp_dim<T, 1>(ctx, c, fw, blk, stream);
+    case 2: return interp_dim<T, 2>(ctx, c, fw, blk, stream);
+    default: return interp_dim<T, 3>(ctx, c, fw, blk, stream);
+  }
+}
+
+template <typename T>
+size_t sm_tile_bytes(int dim, int ns, const int *binsize) {
+  const int H = (ns + 1) / 2;
+  size_t cells = 1;
+  for (int d = 0; d < dim; ++d) cells *= size_t(binsize[d] + 2 * H + 1);
+  return cells * sizeof(cplx_t<T>);
+}
+
+namespace {
+
+int device_lds_limit(int dev) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+      v <= 0)
+    v = 64 * 1024;
+  return std::min(v, 160 * 1024);
+}
+
+template <typename T>
+constexpr int max_ns_of() {
+  return std::is_same<T, float>::value ? MAX_NS_F32 : MAX_NS;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ makeplan
+
+template <typename T>
+Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T tol,
+              const cufinufft_opts &opts)
+    : type_(type), dim_(dim), ntransf_(ntransf), iflag_(iflag >= 0 ? 1 : -1), tol_(tol),
+      opts_(opts) {
+  // Argument checks come first, in the order of makeplan.cu:256-329, and
+  // before any device call.
+  if (type_ < 1 || type_ > 3) fail(FINUFFT_ERR_TYPE_NOTVALID, "invalid type");
+  if (ntransf_ < 1) fail(FINUFFT_ERR_NTRANS_NOTVALID, "invalid ntransf");
+
+  opts_.gpu_maxbatchsize = std::max(opts_.gpu_maxbatchsize, 0);
+  if (opts_.gpu_method != 3) opts_.gpu_np = 0;
+  if (type_ != 3) {
+    for (int d = 0; d < 3; ++d) ms_[d] = d < dim_ ? nmodes[d] : 1;
+    spread_only_ = opts_.gpu_spreadinterponly != 0;
+  } else {
+    spread_only_ = true;  // the outer type-3 stage only spreads
+  }
+  stream_ = static_cast<hipStream_t>(opts_.gpu_stream);
+
+  if (opts_.upsampfac == 0.0) {  // auto (makeplan.cu:289-297)
+    opts_.upsampfac = 2.0;
+    if (type_ == 3 && tol_ >= T(1e-9)) opts_.upsampfac = 1.25;
+  }
+  setup_kernel();
+  if (opts_.gpu_method < 0 || opts_.gpu_method > 4)
+    fail(FINUFFT_ERR_INSUFFICIENT_SHMEM, "invalid gpu_method");
+  if (opts_.gpu_method == 4)
+    fail(FINUFFT_ERR_METHOD_NOTVALID, "block-gather (gpu_method=4) is not provided");
+  if (type_ == 2 && opts_.gpu_method == 3)
+    fail(FINUFFT_ERR_METHOD_NOTVALID, "output-driven (gpu_method=3) only spreads");
+
+  device_guard dg(opts_.gpu_device_id);
+  upload_coefficients();
+  setup_bins();
+
+  if (type_ != 3) {
+    for (int d = 0; d < dim_; ++d) {
+      int64_t n = spread_only_ ? ms_[d] : fine_grid_len(ms_[d], opts_.upsampfac, ns_);
+      if (n > MAX_NF) fail(FINUFFT_ERR_MAXNALLOC, "fine grid exceeds the int32 index range");
+      nf_[d] = int(n);
+    }
+    for (int d = 0; d < dim_; ++d)
+      if (nf_[d] < 2 * ns_)
+        fail(FINUFFT_ERR_SPREAD_BOX_SMALL, "grid dims smaller than 2*nspread");
+    nftot_ = int64_t(nf_[0]) * nf_[1] * nf_[2];
+    nmodes_ = int64_t(ms_[0]) * ms_[1] * ms_[2];
+    if (nftot_ > MAX_NF) fail(FINUFFT_ERR_MAXNALLOC, "fine grid exceeds the int32 index range");
+  }
+  // Vectors per FFT batch: the user's choice, else up to 8 vectors while the
+  // batch grid stays within 2 GiB.
+  if (opts_.gpu_maxbatchsize > 0) {
+    batchsize_ = std::min(opts_.gpu_maxbatchsize, ntransf_);
+  } else {
+    const int64_t per = std::max<int64_t>(nftot_, 1) * int64_t(sizeof(C));
+    batchsize_ = int(std::max<int64_t>(1, std::min<int64_t>({ntransf_, 8, (int64_t(2) << 30) / per})));
+  }
+  if (type_ != 3) prepare_grid();
+}
+
+template <typename T>
+void Plan<T>::setup_kernel() {
+  // setup_spreadinterp (makeplan.cu:43-119)
+  const double sigma = opts_.upsampfac;
+  if (sigma != 2.0 && sigma != 1.25) {
+    if (opts_.gpu_kerevalmeth == 1)
+      fail(FINUFFT_ERR_HORNER_WRONG_BETA,
+           "nonstandard upsampfac cannot use the Horner kernel (kerevalmeth=1)");
+    if (sigma <= 1.0) fail(FINUFFT_ERR_UPSAMPFAC_TOO_SMALL, "upsampfac must exceed 1");
+    if (!opts_.gpu_spreadinterponly && sigma > 4.0 && opts_.debug)
+      std::fprintf(stderr, "[finufft_amd] warning: upsampfac=%.3g is too large to be beneficial\n",
+                   sigma);
+  }
+  T eps = tol_;
+  const T eps_mach = std::numeric_limits<T>::epsilon();
+  if (eps < eps_mach) {
+    if (opts_.debug)
+      std::fprintf(stderr, "[finufft_amd] warning: increasing tol=%.3g to eps_mach=%.3g\n",
+                   double(eps), double(eps_mach));
+    eps = eps_mach;
+  }
+  const KernelShape ks = kernel_shape(double(eps), double(eps / T(10)), sigma, max_ns_of<T>());
+  if (ks.clipped && opts_.debug)
+    std::fprintf(stderr, "[finufft_amd] warning: kernel width clipped to %d\n", ks.ns);
+  ns_ = ks.ns;
+  beta_ = ks.beta;
+  if (opts_.debug)
+    std::printf("[finufft_amd] kerevalmeth=%d tol=%.3g sigma=%.3g: ns=%d beta=%.4g\n",
+                opts_.gpu_kerevalmeth, double(eps), sigma, ns_, beta_);
+}
+
+template <typename T>
+void Plan<T>::upload_coefficients() {
+  if (opts_.gpu_kerevalmeth != 1) return;
+  const int nc = horner_nc_sigma2(ns_);
+  const std::vector<double> c64 = horner_coeffs(ns_, opts_.upsampfac, beta_, nc);
+  std::vector<T> c(c64.begin(), c64.end());
+  coef_.resize(c.size());
+  hip_check(hipMemcpy(coef_.get(), c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice),
+            "hipMemcpy(coef)");
+}
+
+template <typename T>
+void Plan<T>::setup_bins() {
+  // Method choice and bin sizes (makeplan.cu:308-329; heuristics.cu:325-445,
+  // with the LDS rule of this GPU).
+  const bool auto_method = opts_.gpu_method == 0;
+  if (auto_method) opts_.gpu_method = (type_ == 1 || type_ == 3) ? 2 : 1;
+  const int lds_limit = device_lds_limit(opts_.gpu_device_id);
+  int *bs = geom_.binsize;
+  const int user[3] = {opts_.gpu_binsizex, opts_.gpu_binsizey, opts_.gpu_binsizez};
+  auto set_bins = [&](int b) {
+    for (int d = 0; d < 3; ++d) bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : b) : 1;
+  };
+  if (opts_.gpu_method == 1) {
+    set_bins(dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 8);
+  } else {
+    int b = dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 0;
+    if (b == 0) {
+      // Largest cubic bin whose padded tile fits in 128 KiB of LDS.
+      const double cells = double(std::min(lds_limit, 128 * 1024)) / sizeof(C);
+      const int H = (ns_ + 1) / 2;
+      b = int(std::floor(std::cbrt(cells) + 1e-9)) - (2 * H + 1);
+    }
+    set_bins(b);
+    const bool ok = bs[0] >= 1 && bs[1] >= 1 && bs[2] >= 1 &&
+                    sm_tile_bytes<T>(dim_, ns_, bs) <= size_t(lds_limit);
+    if (!ok) {
+      if (auto_method) {
+        opts_.gpu_method = 1;
+        set_bins(dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 8);
+      } else {
+        fail(FINUFFT_ERR_INSUFFICIENT_SHMEM, "LDS tile does not fit for gpu_method 2/3");
+      }
+    } else {
+      lds_bytes_ = sm_tile_bytes<T>(dim_, ns_, bs);
+    }
+  }
+  for (int d = 0; d < 3; ++d)
+    if (bs[d] < 1) fail(FINUFFT_ERR_BINSIZE_NOTVALID, "invalid bin size");
+  if (opts_.gpu_maxsubprobsize < 1) opts_.gpu_maxsubprobsize = 1024;
+  opts_.gpu_binsizex = bs[0];
+  opts_.gpu_binsizey = bs[1];
+  opts_.gpu_binsizez = bs[2];
+  geom_.dim = dim_;
+}
+
+template <typename T>
+void Plan<T>::prepare_grid() {
+  // Bin counts, the work grid, the FFT plan and the kernel's Fourier series.
+  for (int d = 0; d < 3; ++d) {
+    geom_.nf[d] = nf_[d];
+    geom_.nbins[d] = (nf_[d] + geom_.binsize[d] - 1) / geom_.binsize[d];
+  }
+  if (!spread_only_ || type_ == 3) {
+    fw_.resize(size_t(nftot_) * batchsize_);
+  }
+  if (!spread_only_) {
+    fft_.setup(dim_, nf_, batchsize_, std::is_same<T, double>::value);
+    int q = 0;
+    std::vector<double> z(3 * MAX_NQUAD, 0.0), f(3 * MAX_NQUAD, 0.0);
+    for (int d = 0; d < dim_; ++d)
+      fourier_nodes(ns_, beta_, false, &q, z.data() + d * MAX_NQUAD, f.data() + d * MAX_NQUAD);
+    dbuf<double> dz(z.size()), df(f.size());
+    hip_check(hipMemcpy(dz.get(), z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy(nodes)");
+    hip_check(hipMemcpy(df.get(), f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy(weights)");
+    T *outs[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < dim_; ++d) {
+      fwker_[d].resize(size_t(nf_[d] / 2 + 1));
+      outs[d] = fwker_[d].get();
+    }
+    fseries<T>(dim_, nf_, q, df.get(), dz.get(), outs, stream_);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize(fseries)");
+  }
+}
+
+// ------------------------------------------------------------ setpts
+
+template <typename T>
+void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
+  xyz_[0] = x;
+  xyz_[1] = dim_ > 1 ? y : nullptr;
+  xyz_[2] = dim_ > 2 ? z : nullptr;
+  const int nb = geom_.total();
+  idx_.reserve_at_least(std::max(M_, 1));
+  binstart_.reserve_at_least(size_t(nb) + 1);
+  const T *pts[3] = {xyz_[0], xyz_[1], xyz_[2]};
+  if (opts_.gpu_method == 1 && !opts_.gpu_sort) {
+    iota(idx_.get(), M_, stream_);
+  } else {
+    bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_);
+  }
+  if (opts_.gpu_method != 1) {
+    sub_capacity_ = subproblem_capacity(M_, nb, opts_.gpu_maxsubprobsize);
+    substart_.reserve_at_least(size_t(nb) + 1);
+    sub2bin_.reserve_at_least(std::max(sub_capacity_, 1));
+    build_subproblems(binstart_.get(), nb, opts_.gpu_maxsubprobsize, substart_.get(),
+                      sub2bin_.get(), scratch_, stream_);
+  }
+}
+
+template <typename T>
+void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *s, const T *t,
+                     const T *u) {
+  device_guard dg(opts_.gpu_device_id);
+  if (M < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "negative number of points");
+  if (type_ != 3) {
+    M_ = M;
+    sort_points(x, y, z);
+    return;
+  }
+  // ---- type 3 (setpts.cu:92-313)
+  M_ = M;
+  Nk_ = N;
+  if (N < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: negative number of targets");
+  if (int64_t(N) > MAX_NF) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: too many targets");
+  if (dim_ > 0 && s == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: s is null");
+  if (dim_ > 1 && t == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: t is null");
+  if (dim_ > 2 && u == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: u is null");
+  const T *src[3] = {x, y, z};
+  const T *trg[3] = {s, t, u};
+  double h[3] = {1, 1, 1}, gam[3] = {1, 1, 1};
+  int64_t nf123[3] = {1, 1, 1};
+  for (int d = 0; d < dim_; ++d) {
+    double lo, hi;
+    minmax<T>(src[d], M, &lo, &hi, stream_);
+    const auto [X, Cc] = widcen(lo, hi);
+    minmax<T>(trg[d], N, &lo, &hi, stream_);
+    const auto [S, D] = widcen(lo, hi);
+    // The type-3 parameters live in the plan's precision, as in the reference.
+    t3C_[d] = double(T(Cc));
+    t3D_[d] = double(T(D));
+    const auto [nfd, hd, gd] = nhg_type3(opts_.upsampfac, double(T(X)), double(T(S)), ns_, MAX_NF);
+    nf123[d] = nfd;
+    h[d] = double(T(hd));
+    gam[d] = double(T(gd));
+  }
+  const int64_t nft = nf123[0] * nf123[1] * nf123[2];
+  if (nft * std::max(batchsize_, 1) > MAX_NF)
+    fail(FINUFFT_ERR_MAXNALLOC, "type 3: fine grid batch exceeds the int32 index range");
+  for (int d = 0; d < 3; ++d) nf_[d] = int(nf123[d]);
+  nftot_ = nft;
+
+  for (int d = 0; d < dim_; ++d) {
+    xyzp_[d].reserve_at_least(std::max(M, 1));
+    stup_[d].reserve_at_least(std::max(N, 1));
+    const T inv_gam = T(1) / T(gam[d]);
+    affine<T>(src[d], xyzp_[d].get(), M, t3C_[d], double(inv_gam), stream_);
+    const T scale = T(h[d]) * T(gam[d]);
+    affine<T>(trg[d], stup_[d].get(), N, t3D_[d], double(scale), stream_);
+  }
+  prephase_.reserve_at_least(std::max(M, 1));
+  t3_prephase<T>(dim_, src, t3D_, iflag_, M, prephase_.get(), stream_);
+  // phihat at the rescaled targets and the deconvolution factor
+  {
+    int q = 0;
+    std::vector<double> zq(MAX_NQUAD), fq(MAX_NQUAD);
+    fourier_nodes(ns_, beta_, true, &q, zq.data(), fq.data());
+    dbuf<double> dz(MAX_NQUAD), df(MAX_NQUAD);
+    hip_check(hipMemcpy(dz.get(), zq.data(), MAX_NQUAD * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy(t3 nodes)");
+    hip_check(hipMemcpy(df.get(), fq.data(), MAX_NQUAD * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy(t3 weights)");
+    deconv_.reserve_at_least(std::max(N, 1));
+    const T *stup[3] = {stup_[0].get(), stup_[1].get(), stup_[2].get()};
+    t3_deconv<T>(dim_, trg, stup, t3C_, t3D_, iflag_, q, df.get(), dz.get(), N, deconv_.get(),
+                 stream_);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize(t3 deconv)");
+  }
+  prepare_grid();
+  cpre_.reserve_at_least(size_t(std::max(M, 1)) * batchsize_);
+  sort_points(xyzp_[0].get(), xyzp_[1].get(), xyzp_[2].get());
+
+  // inner type-2 plan over the outer fine grid (setpts.cu:295-313); mode
+  // order is forced to CMCL order, which the type-3 algebra assumes (the
+  // reference CPU library does the same, include/finufft/setpts.hpp:298).
+  cufinufft_opts t2opts = opts_;
+  t2opts.gpu_spreadinterponly = 0;
+  t2opts.gpu_method = 0;
+  t2opts.modeord = 0;
+  t2opts.gpu_binsizex = t2opts.gpu_binsizey = t2opts.gpu_binsizez = 0;
+  t2opts.gpu_maxbatchsize = batchsize_;
+  inner_.reset();
+  inner_ = std::make_unique<Plan<T>>(2, dim_, nf_, iflag_, batchsize_, tol_, t2opts);
+  inner_->setpts(N, stup_[0].get(), stup_[1].get(), stup_[2].get(), 0, nullptr, nullptr, nullptr);
+}
+
+// ------------------------------------------------------------ execute
+
+template <typename T>
+SpreadCtx<T> Plan<T>::spread_ctx() const {
+  SpreadCtx<T> c;
+  c.dim = dim_;
+  c.ns = ns_;
+  c.method = opts_.gpu_method == 1 ? 1 : 2;
+  c.M = M_;
+  for (int d = 0; d < 3; ++d) {
+    c.nf[d] = nf_[d];
+    c.xyz[d] = xyz_[d];
+    c.binsize[d] = geom_.binsize[d];
+    c.nbins[d] = geom_.nbins[d];
+  }
+  c.kp.beta = T(beta_);
+  c.kp.es_c = T(4.0 / (double(ns_) * ns_));
+  c.kp.horner = opts_.gpu_kerevalmeth == 1 ? 1 : 0;
+  c.kp.coef = coef_.get();
+  c.idx = idx_.get();
+  c.binstart = binstart_.get();
+  c.maxsub = opts_.gpu_maxsubprobsize;
+  c.substart = substart_.get();
+  c.sub2bin = sub2bin_.get();
+  c.sub_capacity = sub_capacity_;
+  c.lds_bytes = lds_bytes_;
+  return c;
+}
+
+template <typename T>
+void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
+  const SpreadCtx<T> ctx = spread_ctx();
+  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
+  for (int i = 0; i * batchsize_ < ntr; ++i) {
+    const int blk = std::min(ntr - i * batchsize_, batchsize_);
+    const C *c = d_c + int64_t(i) * batchsize_ * M_;
+    C *fk = d_fk + int64_t(i) * batchsize_ * nmodes_;
+    C *fw = spread_only_ ? fk : fw_.get();
+    zero<T>(fw, nftot_ * blk, stream_);
+    spread<T>(ctx, c, fw, blk, stream_);
+    if (spread_only_) continue;
+    fft_.exec(fw, fft_sign, stream_);
+    deconvolve<T>(dim_, opts_.modeord, true, ms_, nf_, ker, fw, fk, blk, stream_);
+  }
+}
+
+template <typename T>
+void Plan<T>::exec_type2(C *d_c, C *d_fk, int fft_sign, int ntr) {
+  const SpreadCtx<T> ctx = spread_ctx();
+  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
+  for (int i = 0; i * batchsize_ < ntr; ++i) {
+    const int blk = std::min(ntr - i * batchsize_, batchsize_);
+    C *c = d_c + int64_t(i) * batchsize_ * M_;
+    C *fk = d_fk + int64_t(i) * batchsize_ * nmodes_;
+    C *fw = fk;
+    if (!spread_only_) {
+      fw = fw_.get();
+      deconvolve<T>(dim_, opts_.modeord, false, ms_, nf_, ker, fw, fk, blk, stream_);
+      fft_.exec(fw, fft_sign, stream_);
+    }
+    interp<T>(ctx, c, fw, blk, stream_);
+  }
+}
+
+template <typename T>
+void Plan<T>::exec_type3(C *d_c, C *d_fk) {
+  const SpreadCtx<T> ctx = spread_ctx();
+  for (int i = 0; i * batchsize_ < ntransf_; ++i) {
+    const int blk = std::min(ntransf_ - i * batchsize_, batchsize_);
+    const C *c = d_c + int64_t(i) * batchsize_ * M_;
+    C *fk = d_fk + int64_t(i) * batchsize_ * Nk_;
+    cmul<T>(c, prephase_.get(), cpre_.get(), M_, blk, stream_);
+    zero<T>(fw_.get(), nftot_ * blk, stream_);
+    spread<T>(ctx, cpre_.get(), fw_.get(), blk, stream_);
+    inner_->exec_type2(fk, fw_.get(), inner_->iflag_, blk);
+    cmul<T>(fk, deconv_.get(), fk, Nk_, blk, stream_);
+  }
+}
+
+template <typename T>
+void Plan<T>::execute(C *c, C *fk) {
+  device_guard dg(opts_.gpu_device_id);
+  switch (type_) {
+    case 1: return exec_type1(c, fk, iflag_, ntransf_);
+    case 2: return exec_type2(c, fk, iflag_, ntransf_);
+    default: return exec_type3(c, fk);
+  }
+}
+
+template <typename T>
+void Plan<T>::execute_adjoint(C *c, C *fk) {
+  device_guard dg(opts_.gpu_device_id);
+  if (type_ == 1) return exec_type2(c, fk, -iflag_, ntransf_);
+  if (type_ == 2) return exec_type1(c, fk, -iflag_, ntransf_);
+  fail(FINUFFT_ERR_TYPE_NOTVALID, "adjoint is provided for types 1 and 2");
+}
+
+template class Plan<float>;
+template class Plan<double>;
+
+}  // namespace fa

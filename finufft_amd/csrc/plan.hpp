@@ -1,0 +1,115 @@
+// The NUFFT plan: kernel and grid choice at construction, point sort at
+// setpts, spread/interp -> FFT -> deconvolve at execute.
+//
+// Mirrors the life cycle of the reference's cufinufft_plan_t
+// (include/cufinufft/cufinufft_plan_t.hpp:205-499; src/cuda/makeplan.cu:223-398,
+// setpts.cu:77-314, execute.cu:105-271) with these differences of form:
+//   * the bin sort is deterministic (stable radix sort, sort.hpp);
+//   * setpts for types 1/2 never synchronizes with the host (the subproblem
+//     table is sized by an upper bound and its true length stays on the GPU);
+//   * the fine-grid work buffer is allocated once at plan time, not at every
+//     execute, and grid clearing uses a kernel rather than a memset node, so
+//     execute is capturable in a hipGraph;
+//   * all ntransf vectors of a batch go through one launch per stage.
+#pragma once
+
+#include <memory>
+
+#include <cufinufft_opts.h>
+
+#include "common.hpp"
+#include "fft.hpp"
+#include "host_math.hpp"
+#include "sort.hpp"
+#include "spreadinterp.hpp"
+
+namespace fa {
+
+template <typename T>
+class Plan {
+ public:
+  using C = cplx_t<T>;
+
+  Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T tol,
+       const cufinufft_opts &opts);
+
+  void setpts(int M, const T *x, const T *y, const T *z, int N, const T *s, const T *t,
+              const T *u);
+  // The plan's transform (type 1, 2 or 3).
+  void execute(C *c, C *fk);
+  // Adjoint of the plan's transform on the same points: type 1 -> type 2 with
+  // the opposite sign and vice versa (the reference CPU library's
+  // execute_adjoint, include/finufft/execute.hpp:318-357). Types 1 and 2 only.
+  void execute_adjoint(C *c, C *fk);
+
+  // introspection (tests, bench)
+  int type() const { return type_; }
+  int dim() const { return dim_; }
+  int ns() const { return ns_; }
+  double beta() const { return beta_; }
+  double upsampfac() const { return opts_.upsampfac; }
+  int method() const { return opts_.gpu_method; }
+  int nf(int d) const { return nf_[d]; }
+  int binsize(int d) const { return geom_.binsize[d]; }
+  int batchsize() const { return batchsize_; }
+  int M() const { return M_; }
+  const int *sort_perm() const { return idx_.get(); }
+  const int *bin_start() const { return binstart_.get(); }
+  int nbins_total() const { return geom_.total(); }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void setup_kernel();
+  void upload_coefficients();
+  void setup_bins();
+  void sort_points(const T *x, const T *y, const T *z);
+  SpreadCtx<T> spread_ctx() const;
+  void exec_type1(C *c, C *fk, int fft_sign, int ntr);
+  void exec_type2(C *c, C *fk, int fft_sign, int ntr);
+  void exec_type3(C *c, C *fk);
+  void prepare_grid();
+
+  // parameters
+  int type_, dim_, ntransf_, iflag_;
+  T tol_;
+  cufinufft_opts opts_;
+  int batchsize_ = 1;
+  hipStream_t stream_ = nullptr;
+  bool spread_only_ = false;
+
+  // kernel
+  int ns_ = 2;
+  double beta_ = 0;
+  dbuf<T> coef_;
+
+  // sizes
+  int ms_[3] = {1, 1, 1};
+  int nf_[3] = {1, 1, 1};
+  int64_t nftot_ = 1;
+  int64_t nmodes_ = 1;
+  int M_ = 0;
+  int Nk_ = 0;  // type-3 targets
+  const T *xyz_[3] = {nullptr, nullptr, nullptr};
+
+  // kernel Fourier series (types 1/2)
+  dbuf<T> fwker_[3];
+
+  // sort / subproblems
+  BinGeom geom_;
+  dbuf<int> idx_, binstart_, substart_, sub2bin_;
+  int sub_capacity_ = 0;
+  SortScratch scratch_;
+  size_t lds_bytes_ = 0;
+
+  // fine grid work buffer and FFT
+  dbuf<C> fw_;
+  FftPlan fft_;
+
+  // type 3
+  dbuf<T> xyzp_[3], stup_[3];
+  dbuf<C> prephase_, deconv_, cpre_;
+  double t3C_[3] = {0, 0, 0}, t3D_[3] = {0, 0, 0};
+  std::unique_ptr<Plan<T>> inner_;
+};
+
+}  // namespace fa

@@ -1,0 +1,57 @@
+// Element-wise and small kernels of the plan: kernel Fourier series,
+// deconvolution / amplification between the fine grid and the modes, the
+// type-3 rescale / phase factors, min/max reductions and grid zeroing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "spreadinterp.hpp"
+
+namespace fa {
+
+// Up to three 1-D series of the kernel's Fourier transform at integer
+// frequencies i = 0..nf_d/2 (type 1/2):
+//   out_d[i] = (-1)^i sum_n f_n 2 cos(2 pi i z_n / nf_d)
+// (src/cuda/fft.cu:29-49; the sign flip places the grid origin at x = -pi).
+// f and z hold q nodes per dimension at stride MAX_NQUAD; evaluated in double.
+template <typename T>
+void fseries(int dim, const int nf[3], int q, const double *f, const double *z, T *const out[3],
+             hipStream_t stream);
+
+// fk <-> fw for blk vectors. Type 1 (to_modes): fk[k] = fw[w(k)] / prod_d
+// ker_d[|k_d|]. Type 2: fw[w(k)] = fk[k] / prod_d ker_d[|k_d|] and every other
+// fw cell is set to 0 (so no separate clearing pass is needed).
+// modeord 0: k from -N/2 upward; 1: FFT order (src/cuda/execute.cu:24-63).
+template <typename T>
+void deconvolve(int dim, int modeord, bool to_modes, const int ms[3], const int nf[3],
+                const T *const ker[3], cplx_t<T> *fw, cplx_t<T> *fk, int blk,
+                hipStream_t stream);
+
+// Sets n complex values to zero (graph-capture friendly, no memset node).
+template <typename T>
+void zero(cplx_t<T> *p, int64_t n, hipStream_t stream);
+
+// lo/hi of a device array (n >= 1); synchronizes the stream.
+template <typename T>
+void minmax(const T *a, int n, double *lo, double *hi, hipStream_t stream);
+
+// ----- type 3 (src/cuda/setpts.cu:172-293) -----
+// out[i] = (in[i] - center) * inv_scale
+template <typename T>
+void affine(const T *in, T *out, int n, double center, double scale, hipStream_t stream);
+// pre[j] = exp(i sign (D . x_j)); phases accumulated in double.
+template <typename T>
+void t3_prephase(int dim, const T *const xyz[3], const double D[3], int sign, int M,
+                 cplx_t<T> *pre, hipStream_t stream);
+// dec[k] = exp(i sign C.(s_k - D)) / prod_d phihat_d(s'_k), with
+// phihat(s) = sum_n f_n 2 cos(s z_n) (fft.cu:57-76); C = 0 skips the phase.
+template <typename T>
+void t3_deconv(int dim, const T *const stu[3], const T *const stup[3], const double C[3],
+               const double D[3], int sign, int q, const double *f, const double *z, int N,
+               cplx_t<T> *dec, hipStream_t stream);
+// a[t*n + i] *= b[i] for t < blk, or out[t*n + i] = a[t*n + i] * b[i].
+template <typename T>
+void cmul(const cplx_t<T> *a, const cplx_t<T> *b, cplx_t<T> *out, int n, int blk,
+          hipStream_t stream);
+
+}  // namespace fa

@@ -1,0 +1,51 @@
+// Deterministic bin sort of the nonuniform points and the subproblem table.
+//
+// The reference GPU sort (calc_bin_size_noghost + exclusive scan +
+// calc_inverse_of_global_sort_idx, include/cufinufft/spreadinterp.hpp:299-325)
+// orders points by bin but leaves the order inside a bin to atomicAdd
+// arrival. This sort is a stable LSD radix sort on the same bin key, so its
+// permutation is fully determined: bins in x-fastest order, points inside a
+// bin by increasing original index (the order of the reference's CPU
+// bin_sort_singlethread_impl, include/finufft/spread.hpp:459-584).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace fa {
+
+struct BinGeom {
+  int dim = 1;
+  int nf[3] = {1, 1, 1};
+  int binsize[3] = {1, 1, 1};
+  int nbins[3] = {1, 1, 1};
+  int total() const { return nbins[0] * nbins[1] * nbins[2]; }
+};
+
+// Scratch reused across setpts calls (grows only).
+struct SortScratch {
+  dbuf<int> keys_a, keys_b, vals_b, hist, scan_tmp;
+};
+
+// idx[M]: sorted point order. binstart[nbins+1]: first sorted position of each
+// bin, binstart[nbins] = M. Nothing is synchronized with the host.
+template <typename T>
+void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *binstart,
+              SortScratch &s, hipStream_t stream);
+
+// idx[i] = i.
+void iota(int *idx, int M, hipStream_t stream);
+
+// substart[nbins+1] (exclusive scan of ceil(count/maxsub), last entry = total)
+// and sub2bin[capacity], capacity = subproblem_capacity(M, nbins, maxsub).
+void build_subproblems(const int *binstart, int nbins, int maxsub, int *substart,
+                       int *sub2bin, SortScratch &s, hipStream_t stream);
+inline int subproblem_capacity(int M, int nbins, int maxsub) {
+  return nbins + (M + maxsub - 1) / maxsub;
+}
+
+// out[i] = sum_{k<i} in[k] for i in [0, n); in and out may alias.
+void exclusive_scan(const int *in, int *out, int n, SortScratch &s, hipStream_t stream);
+
+}  // namespace fa

@@ -1,0 +1,77 @@
+// Interface between the plan and the spreading / interpolation kernels.
+// The kernels themselves live in spreadinterp.hip and are instantiated once
+// per (precision, dimension) translation unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernel_math.hpp"
+
+namespace fa {
+
+template <typename T>
+struct cplx_of;
+template <>
+struct cplx_of<float> {
+  using type = float2;
+};
+template <>
+struct cplx_of<double> {
+  using type = double2;
+};
+template <typename T>
+using cplx_t = typename cplx_of<T>::type;
+
+// Everything a spread or interp launch needs about the points, their sorted
+// order and the fine grid. All pointers are device pointers.
+template <typename T>
+struct SpreadCtx {
+  int dim = 1;
+  int ns = 2;
+  int method = 1;  // 1: point driven (GM / GM-sort), 2: LDS subproblems (SM)
+  int M = 0;
+  int nf[3] = {1, 1, 1};
+  const T *xyz[3] = {nullptr, nullptr, nullptr};
+  KerParams<T> kp{};
+
+  const int *idx = nullptr;  // point order (sorted or identity), length M
+
+  // bin geometry of the sort (x fastest)
+  int binsize[3] = {1, 1, 1};
+  int nbins[3] = {1, 1, 1};
+  const int *binstart = nullptr;  // nbins_tot + 1 entries, binstart[nbins_tot] = M
+
+  // subproblems (method 2): subproblem s covers points
+  // idx[binstart[b] + (s - substart[b]) * maxsub ...] of bin b = sub2bin[s]
+  int maxsub = 1024;
+  const int *substart = nullptr;  // nbins_tot + 1 entries, last = total count
+  const int *sub2bin = nullptr;
+  int sub_capacity = 0;  // launch grid (>= the device-side total)
+  size_t lds_bytes = 0;  // LDS tile bytes of the method-2 kernels
+};
+
+// c (M x blk, contiguous per vector) -> fw (nf_tot x blk). fw must be zeroed
+// by the caller. Dispatches on ctx.dim / ctx.ns / ctx.method.
+template <typename T>
+void spread(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
+            hipStream_t stream);
+
+// fw (nf_tot x blk) -> c (M x blk).
+template <typename T>
+void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
+            hipStream_t stream);
+
+// Per-dimension entry points (one translation unit each).
+template <typename T, int DIM>
+void spread_dim(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
+                hipStream_t stream);
+template <typename T, int DIM>
+void interp_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
+                hipStream_t stream);
+
+// LDS bytes the method-2 tile of this geometry needs:
+// prod_d (binsize_d + 2 ceil(ns/2) + 1) complex values.
+template <typename T>
+size_t sm_tile_bytes(int dim, int ns, const int *binsize);
+
+}  // namespace fa

@@ -1,0 +1,178 @@
+"""GPU parity of the NUFFT against the FP64 direct sum (the oracle).
+
+Mirrors the reference's GPU test matrix (test/cuda/CMakeLists.txt:26-250 and
+python/cufinufft/tests/test_basic.py): dimensions 1-3, types 1-3, both
+precisions, the point-driven (1) and LDS subproblem (2) methods, with the
+pass criteria of python/cufinufft/tests/utils.py:121-168 (25 tol for types
+1/2, 100 tol for type 3), evaluated on the full output of small problems.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import nudft
+from tests.helpers import REAL, cnormal, rng, to_dev, to_host, uniform_pts
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {1: (97,), 2: (24, 31), 3: (10, 13, 16)}
+TOL = {"complex64": 1e-5, "complex128": 1e-11}
+
+
+def _run(torch, ttype, shape, dtype, method, M=1500, ntr=1, isign=None, seed=0, **kw):
+    from finufft_amd import Plan
+
+    r = rng(seed)
+    dim = len(shape)
+    real = REAL[dtype]
+    pts = uniform_pts(r, dim, M, real)
+    tol = TOL[dtype]
+    plan = Plan(ttype, shape, n_trans=ntr, eps=tol, isign=isign, dtype=dtype,
+                gpu_method=method, **kw)
+    plan.setpts(*[to_dev(torch, p) for p in pts])
+    sign = plan._isign
+    if ttype == 1:
+        c = cnormal(r, (ntr, M) if ntr > 1 else (M,), np.dtype(dtype))
+        f = to_host(plan.execute(to_dev(torch, c)))
+        ref = nudft.type1([p.astype(np.float64) for p in pts], c, sign, shape,
+                          modeord=kw.get("modeord", 0))
+        return nudft.relerr(f, ref), tol
+    f = cnormal(r, ((ntr,) if ntr > 1 else ()) + tuple(shape), np.dtype(dtype))
+    c = to_host(plan.execute(to_dev(torch, f)))
+    ref = nudft.type2([p.astype(np.float64) for p in pts], f, sign, modeord=kw.get("modeord", 0))
+    return nudft.relerr(c, ref), tol
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("ttype", [1, 2])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("method", [1, 2])
+def test_type12_vs_direct(torch_gpu, dim, ttype, dtype, method):
+    err, tol = _run(torch_gpu, ttype, SHAPES[dim], dtype, method)
+    print(f"dim={dim} type={ttype} {dtype} method={method}: relerr={err:.3e} tol={tol}")
+    assert err < 25 * tol
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_type3_vs_direct(torch_gpu, dim, dtype):
+    from finufft_amd import Plan
+
+    torch = torch_gpu
+    r = rng(3)
+    real = REAL[dtype]
+    M, N = 1200, 900
+    tol = TOL[dtype]
+    x = [r.uniform(-2.0, 3.0, M).astype(real) for _ in range(dim)]  # off-centre sources
+    s = [r.uniform(-40.0, 25.0, N).astype(real) for _ in range(dim)]
+    c = cnormal(r, (M,), np.dtype(dtype))
+    plan = Plan(3, dim, eps=tol, dtype=dtype)
+    plan.setpts(*[to_dev(torch, a) for a in x], *[to_dev(torch, a) for a in s])
+    f = to_host(plan.execute(to_dev(torch, c)))
+    ref = nudft.type3([a.astype(np.float64) for a in x], c, 1, [a.astype(np.float64) for a in s])
+    err = nudft.relerr(f, ref)
+    print(f"type3 dim={dim} {dtype}: relerr={err:.3e}")
+    assert err < 100 * tol
+
+
+@pytest.mark.parametrize("ttype", [1, 2])
+def test_many_vectors(torch_gpu, ttype):
+    err, tol = _run(torch_gpu, ttype, (20, 22), "complex64", 2, M=1000, ntr=5,
+                    gpu_maxbatchsize=2)
+    assert err < 25 * tol
+
+
+@pytest.mark.parametrize("ttype", [1, 2])
+def test_modeord_fft_order(torch_gpu, ttype):
+    err, tol = _run(torch_gpu, ttype, (12, 15, 9), "complex128", 2 if ttype == 1 else 1,
+                    modeord=1)
+    assert err < 25 * tol
+
+
+@pytest.mark.parametrize("upsampfac,kerevalmeth", [(1.25, 1), (1.25, 0), (2.0, 0), (1.5, 0)])
+def test_upsampfac_and_direct_kernel(torch_gpu, upsampfac, kerevalmeth):
+    for ttype in (1, 2):
+        err, tol = _run(torch_gpu, ttype, (30, 26), "complex128", 2 if ttype == 1 else 1,
+                        upsampfac=upsampfac, gpu_kerevalmeth=kerevalmeth)
+        print(f"sigma={upsampfac} kerevalmeth={kerevalmeth} type={ttype}: {err:.3e}")
+        assert err < 25 * tol
+
+
+def test_tol_1e6_fp32_3d_meets_tol(torch_gpu):
+    """The 3-D fp32 configuration at tol 1e-6 stays within the requested tol."""
+    for ttype in (1, 2):
+        for method in (1, 2):
+            from finufft_amd import Plan
+
+            torch = torch_gpu
+            r = rng(11)
+            shape = (32, 32, 32)
+            M = 20000
+            pts = uniform_pts(r, 3, M, np.float32)
+            plan = Plan(ttype, shape, eps=1e-6, dtype="complex64", gpu_method=method)
+            plan.setpts(*[to_dev(torch, p) for p in pts])
+            p64 = [p.astype(np.float64) for p in pts]
+            if ttype == 1:
+                c = cnormal(r, (M,), np.complex64)
+                f = to_host(plan.execute(to_dev(torch, c)))
+                idx = r.choice(np.prod(shape), 2000, replace=False)
+                ref = nudft.type1_at(p64, c, 1, shape, idx)
+                err = nudft.relerr(f.ravel()[idx], ref)
+            else:
+                f = cnormal(r, shape, np.complex64)
+                c = to_host(plan.execute(to_dev(torch, f)))
+                idx = r.choice(M, 2000, replace=False)
+                ref = nudft.type2(p64, f, -1, idx=idx)
+                err = nudft.relerr(c[idx], ref)
+            print(f"3D fp32 tol 1e-6 type {ttype} method {method}: relerr {err:.3e}")
+            assert err <= 1e-6
+
+
+def test_adjointness(torch_gpu):
+    """<type1(c), f> = <c, type2(f)> for the plan and its adjoint (test/adjointness.cpp)."""
+    from finufft_amd import Plan
+
+    torch = torch_gpu
+    r = rng(5)
+    shape = (16, 18, 20)
+    M = 3000
+    pts = uniform_pts(r, 3, M, np.float64)
+    plan = Plan(1, shape, eps=1e-12, dtype="complex128")
+    plan.setpts(*[to_dev(torch, p) for p in pts])
+    c = cnormal(r, (M,))
+    f = cnormal(r, shape)
+    F = to_host(plan.execute(to_dev(torch, c)))
+    C = to_host(plan.execute_adjoint(to_dev(torch, f)))
+    lhs = np.vdot(f.ravel(), F.ravel())
+    rhs = np.vdot(C, c)
+    denom = np.linalg.norm(f) * np.linalg.norm(F) / np.sqrt(np.prod(shape))
+    assert abs(lhs - rhs) / denom < 1e-10
+
+
+def test_sort_permutation_bit_exact(torch_gpu):
+    """The sort is the stable bin order of the reference's bin index."""
+    import ctypes
+
+    from finufft_amd import Plan, _lib
+    from oracle import sort as osort
+
+    torch = torch_gpu
+    r = rng(7)
+    M = 200_000
+    pts = uniform_pts(r, 3, M, np.float32)
+    # edge coordinates: the box ends, far outside, and just below +pi
+    pts[0][:6] = np.array([-np.pi, np.pi, 3 * np.pi, -7.5, np.nextafter(np.float32(np.pi), 0),
+                           0.0], np.float32)
+    for method in (1, 2):
+        plan = Plan(1, (64, 48, 40), eps=1e-6, dtype="complex64", gpu_method=method)
+        plan.setpts(*[to_dev(torch, p) for p in pts])
+        info = plan.info()
+        perm = torch.empty(M, dtype=torch.int32, device="cuda")
+        binstart = torch.empty(info["nbins"] + 1, dtype=torch.int32, device="cuda")
+        ier = _lib.lib().finufft_amd_copy_sort(plan.handle, 0, ctypes.c_void_p(perm.data_ptr()),
+                                               ctypes.c_void_p(binstart.data_ptr()))
+        assert ier == 0
+        pts_c = pts[::-1]  # x-fastest C order
+        ref_perm, ref_start = osort.bin_sort(pts_c, list(info["nf"]), list(info["binsize"]))
+        assert np.array_equal(to_host(perm), ref_perm), f"method {method}"
+        assert np.array_equal(to_host(binstart), ref_start), f"method {method}"
