@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the NUFFT hot path on one MI355X.
+
+Workload (the headline configuration): 3-D type 1 and type 2, fp32
+(complex64 data), tol 1e-6, M = 1e8 nonuniform points i.i.d. uniform on
+[-pi, pi)^3, N = 256^3 modes, complex N(0,1) strengths / coefficients,
+isign +1 (type 1) and -1 (type 2). Inputs are generated on the GPU (seeded)
+and resident in HBM before timing. Plans use the library's default options
+(the reference's method heuristic: LDS subproblems for type 1, point-driven
+for type 2).
+
+One step = one type-1 execute + one type-2 execute over the same points.
+value = 2 M / (time per step) NU pts/s. setpts (the sort) is timed separately
+and reported, not counted.
+
+Extra fields:
+  roofline      the dominant kernel (longest average launch among spread and
+                interp), algorithmic bytes per launch over its average launch
+                time, timed with HIP events on the plan's stream over the
+                timed region;
+  stages        average ms per execute of every stage of both plans;
+  error         relative l2 error against an FP64 direct sum on the GPU at
+                sampled modes (type 1) and sampled points (type 2);
+  cpu_baseline  the oracle's CPU restatement (C + OpenMP spread/interp,
+                scipy.fft FFT) on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--M 1e8] [--N 256]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "NU pts/s 3D type1/2 fp32 @tol1e-6; batched+learned vs plain speedup, same err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(N, tol, M_full, M_sample, threads):
+    """Times the oracle's CPU restatement on a sample of the workload and
+    scales the spread/interp part to the full point count."""
+    import numpy as np
+
+    from oracle import cpu
+
+    r = np.random.default_rng(0)
+    pts = [r.uniform(-np.pi, np.pi, M_sample).astype(np.float32) for _ in range(3)]
+    c = (r.standard_normal(M_sample) + 1j * r.standard_normal(M_sample)).astype(np.complex64)
+    f = (r.standard_normal((N, N, N)) + 1j * r.standard_normal((N, N, N))).astype(np.complex64)
+    out = {}
+    for ttype, data in ((1, c), (2, f)):
+        p = cpu.CpuPlan(ttype, (N, N, N), eps=tol, nthreads=threads)
+        p.setpts(*pts)
+        p.execute(data)  # warm (page-in, FFT plan)
+        t0 = time.perf_counter()
+        p.execute(data)
+        total = time.perf_counter() - t0
+        # fixed cost (grid FFT + deconvolution) from a run with no points
+        pz = cpu.CpuPlan(ttype, (N, N, N), eps=tol, nthreads=threads)
+        pz.setpts(*[a[:1] for a in pts])
+        t0 = time.perf_counter()
+        pz.execute(data[:1] if ttype == 1 else data)
+        fixed = time.perf_counter() - t0
+        per_pt = max(total - fixed, 0.0) / M_sample
+        out[ttype] = fixed + per_pt * M_full
+    value = 2 * M_full / (out[1] + out[2])
+    return {
+        "value": value,
+        "unit": "NU pts/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"type1+type2 at N={N}^3, tol {tol}, on M={M_sample:.0e} of the {M_full:.0e} "
+                   f"points; spread/interp time scaled linearly to M={M_full:.0e}, grid FFT "
+                   f"(scipy.fft) and deconvolution timed at full size; projected s/step "
+                   f"t1={out[1]:.2f} t2={out[2]:.2f}"),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--M", type=float, default=1e8)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--cpu-sample", type=float, default=2e6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--err-samples", type=int, default=256)
+    ap.add_argument("--method1", type=int, default=0, help="gpu_method of the type-1 plan (0 auto)")
+    ap.add_argument("--method2", type=int, default=0, help="gpu_method of the type-2 plan (0 auto)")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from finufft_amd import Plan
+    from finufft_amd import accuracy as acc
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    M, N, tol = int(args.M), args.N, args.tol
+    shape = (N, N, N)
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    pts = [(torch.rand(M, generator=g, device=dev, dtype=torch.float32) * (2 * math.pi) - math.pi)
+           for _ in range(3)]
+    g.manual_seed(1)
+    c = torch.complex(torch.randn(M, generator=g, device=dev),
+                      torch.randn(M, generator=g, device=dev))
+    f = torch.complex(torch.randn(shape, generator=g, device=dev),
+                      torch.randn(shape, generator=g, device=dev))
+    fk_out = torch.empty(shape, dtype=torch.complex64, device=dev)
+    c_out = torch.empty(M, dtype=torch.complex64, device=dev)
+    torch.cuda.synchronize()
+    log(f"inputs ready: M={M:.3g} N={N}^3")
+
+    p1 = Plan(1, shape, eps=tol, dtype="complex64", isign=1, gpu_method=args.method1)
+    p2 = Plan(2, shape, eps=tol, dtype="complex64", isign=-1, gpu_method=args.method2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p1.setpts(*pts)
+    torch.cuda.synchronize()
+    t_set1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    p2.setpts(*pts)
+    torch.cuda.synchronize()
+    t_set2 = time.perf_counter() - t0
+    info1, info2 = p1.info(), p2.info()
+    log(f"setpts: {t_set1*1e3:.1f} ms / {t_set2*1e3:.1f} ms; plan1 {info1}; plan2 {info2}")
+
+    for _ in range(args.warmup):
+        p1.execute(c, out=fk_out)
+        p2.execute(f, out=c_out)
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    p1.set_stage_timing(True)
+    p2.set_stage_timing(True)
+    p1.stage_times()
+    p2.stage_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p1.execute(c, out=fk_out)
+        p2.execute(f, out=c_out)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s1 = {k: v / args.steps for k, v in p1.stage_times().items()}
+    s2 = {k: v / args.steps for k, v in p2.stage_times().items()}
+    p1.set_stage_timing(False)
+    p2.set_stage_timing(False)
+    ms_step = elapsed / args.steps * 1e3
+    value = 2 * M / (elapsed / args.steps)
+    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step; type1 stages {s1}; type2 stages {s2}")
+
+    # roofline of the dominant kernel
+    nf1 = math.prod(info1["nf"])
+    nf2 = math.prod(info2["nf"])
+    spread_bytes = M * (4 + 4 * 3 + 8) + 8 * nf1
+    interp_bytes = 8 * nf2 + M * (4 + 4 * 3) + 8 * M
+    cand = [("spread (type 1)", s1["spread"], spread_bytes), ("interp (type 2)", s2["interp"], interp_bytes)]
+    name, ms, nbytes = max(cand, key=lambda t: t[1])
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": name, "ms_per_launch": round(ms, 4), "alg_bytes": nbytes}
+
+    # accuracy against the FP64 direct sum at sampled modes / points
+    r = torch.Generator(device="cpu")
+    r.manual_seed(2)
+    S = args.err_samples
+    midx = torch.randint(0, N ** 3, (S,), generator=r)
+    pidx = torch.randint(0, M, (S,), generator=r)
+    ref1 = acc.type1_at(pts, c, 1, shape, midx)
+    e1 = acc.relerr(fk_out.reshape(-1)[midx.to(dev)], ref1)
+    ref2 = acc.type2_at(pts, f, -1, pidx)
+    e2 = acc.relerr(c_out[pidx.to(dev)], ref2)
+    log(f"relerr type1 {e1:.3e} type2 {e2:.3e}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline on {threads} threads ...")
+        cpu = cpu_baseline(N, tol, M, int(args.cpu_sample), threads)
+        log(f"cpu baseline {cpu['value']:.3e} NU pts/s")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "NU pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded uniform points, complex normal data, generated on the GPU)",
+        "config": {
+            "workload": f"3D type1+type2 fp32 tol {tol:g}, M={M:.0e} uniform on [-pi,pi)^3, "
+                        f"N={N}^3 modes",
+            "M": M, "N": [N, N, N], "tol": tol,
+            "type1": {"gpu_method": info1["gpu_method"], "binsize": info1["binsize"],
+                      "ns": info1["ns"], "nf": info1["nf"]},
+            "type2": {"gpu_method": info2["gpu_method"], "binsize": info2["binsize"],
+                      "ns": info2["ns"], "nf": info2["nf"]},
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_ms": {"type1": {k: round(v, 4) for k, v in s1.items()},
+                      "type2": {k: round(v, 4) for k, v in s2.items()}},
+        "setpts_ms": {"type1": round(t_set1 * 1e3, 2), "type2": round(t_set2 * 1e3, 2)},
+        "error": {"type1": e1, "type2": e2, "samples": S},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

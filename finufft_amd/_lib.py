@@ -75,6 +75,10 @@ def _declare(lib):
     lib.finufft_amd_plan_info.restype = c_int
     lib.finufft_amd_copy_sort.argtypes = [vp, c_int, vp, vp]
     lib.finufft_amd_copy_sort.restype = c_int
+    lib.finufft_amd_set_stage_timing.argtypes = [vp, c_int, c_int]
+    lib.finufft_amd_set_stage_timing.restype = c_int
+    lib.finufft_amd_get_stage_times.argtypes = [vp, c_int, POINTER(c_double)]
+    lib.finufft_amd_get_stage_times.restype = c_int
     lib.finufft_amd_next235.argtypes = [c_int64, c_int64]
     lib.finufft_amd_next235.restype = c_int64
     lib.finufft_amd_gaussquad.argtypes = [c_int, POINTER(c_double), POINTER(c_double)]

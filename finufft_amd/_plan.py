@@ -148,6 +148,22 @@ class Plan:
             "beta": dout[0], "upsampfac": dout[1],
         }
 
+    STAGES = ("zero", "spread", "fft", "deconvolve", "interp", "type3")
+
+    def set_stage_timing(self, on: bool = True):
+        """Record HIP events around every execute stage (see stage_times)."""
+        ier = _lib.lib().finufft_amd_set_stage_timing(self._plan, int(self._double), int(on))
+        if ier != 0:
+            raise FinufftError("set_stage_timing", ier)
+
+    def stage_times(self) -> dict:
+        """Milliseconds per stage since the last call (waits for the GPU)."""
+        ms = (c_double * 8)()
+        ier = _lib.lib().finufft_amd_get_stage_times(self._plan, int(self._double), ms)
+        if ier != 0:
+            raise FinufftError("get_stage_times", ier)
+        return {k: ms[i] for i, k in enumerate(self.STAGES)}
+
     def _check(self, t, name, real, output=False):
         if t is None:
             return None

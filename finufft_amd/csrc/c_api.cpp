@@ -213,6 +213,25 @@ int finufft_amd_copy_sort(const void *plan, int is_double, int *perm, int *binst
   });
 }
 
+int finufft_amd_set_stage_timing(void *plan, int is_double, int on) {
+  return safe_call([&] {
+    if (is_double)
+      as_plan<double>(plan)->timer().enable(on != 0);
+    else
+      as_plan<float>(plan)->timer().enable(on != 0);
+  });
+}
+
+int finufft_amd_get_stage_times(void *plan, int is_double, double *ms) {
+  return safe_call([&] {
+    if (!ms) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "null output");
+    if (is_double)
+      as_plan<double>(plan)->timer().collect(ms);
+    else
+      as_plan<float>(plan)->timer().collect(ms);
+  });
+}
+
 int64_t finufft_amd_next235(int64_t n, int64_t f) { return fa::next235(n, f); }
 
 void finufft_amd_gaussquad(int n, double *x, double *w) {

@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include <finufft_errors.h>
 
@@ -122,6 +123,71 @@ class device_guard {
 
  private:
   int prev_ = 0, dev_ = 0;
+};
+
+// Per-stage GPU timing with HIP events recorded on the plan's stream.
+// Disabled by default; when enabled, every stage of execute is bracketed by a
+// pair of events, and collect() waits for them and adds up the milliseconds.
+enum Stage { ST_ZERO = 0, ST_SPREAD, ST_FFT, ST_DECONV, ST_INTERP, ST_T3, ST_COUNT = 8 };
+
+class StageTimer {
+ public:
+  StageTimer() = default;
+  StageTimer(const StageTimer &) = delete;
+  StageTimer &operator=(const StageTimer &) = delete;
+  ~StageTimer() {
+    for (auto &r : recs_) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+  }
+  void enable(bool on) { on_ = on; }
+  bool enabled() const { return on_; }
+  void begin(int stage, hipStream_t s) {
+    if (!on_) return;
+    cur_ = Rec{stage, take(), take()};
+    hip_check(hipEventRecord(cur_.a, s), "hipEventRecord");
+  }
+  void end(hipStream_t s) {
+    if (!on_) return;
+    hip_check(hipEventRecord(cur_.b, s), "hipEventRecord");
+    recs_.push_back(cur_);
+  }
+  // Adds the elapsed time of every recorded stage into ms[ST_COUNT] and
+  // recycles the events.
+  void collect(double *ms) {
+    for (int i = 0; i < ST_COUNT; ++i) ms[i] = 0;
+    for (auto &r : recs_) {
+      hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+      float t = 0;
+      hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
+      ms[r.stage] += t;
+      pool_.push_back(r.a);
+      pool_.push_back(r.b);
+    }
+    recs_.clear();
+  }
+
+ private:
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  hipEvent_t take() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  bool on_ = false;
+  Rec cur_{};
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> pool_;
 };
 
 }  // namespace fa

@@ -100,16 +100,17 @@ struct KerParams {
   T beta;         // ES shape parameter
   T es_c;         // (2/ns)^2
   int horner;     // 1: Horner polynomial, 0: direct exp(sqrt())
+  int fold64;     // fp32 plans: fold the coordinate in fp64 (see start_and_kernel)
   const T *coef;  // device: coef[k*ns + i], k = 0..NC(ns)-1, ascending powers
 };
 
 // Fills ker[i] = phi at the ns grid offsets for a point whose first grid node
-// sits at xl = s - x_rescaled in [-ns/2, -ns/2 + 1].
+// sits at xl = s - x_rescaled in [-ns/2, -ns/2 + 1]; z = 2 xl + ns - 1 is the
+// same offset scaled to [-1, 1] (the Horner variable).
 template <typename T, int NS>
-__device__ __forceinline__ void eval_kernel(T *ker, T xl, const KerParams<T> &kp) {
+__device__ __forceinline__ void eval_kernel(T *ker, T xl, T z, const KerParams<T> &kp) {
   if (kp.horner) {
     constexpr int NC = horner_nc_sigma2(NS);
-    const T z = std::fma(T(2), xl, T(NS - 1));
 #pragma unroll
     for (int i = 0; i < NS; ++i) ker[i] = kp.coef[(NC - 1) * NS + i];
 #pragma unroll
@@ -129,15 +130,49 @@ __device__ __forceinline__ void eval_kernel(T *ker, T xl, const KerParams<T> &kp
 
 // Start node and kernel values for one coordinate: s = ceil(x - ns/2)
 // (spreadinterp.hpp:154-196).
+//
+// fp32 plans with kp.fold64 fold the coordinate in fp64 and round only the
+// in-cell offset to fp32. The fp32 fold of the reference keeps ~6e-8 of the
+// period, a phase error of about 2 pi k 6e-8 at mode k (1e-5 at k = 128),
+// which alone exceeds a 1e-6 tolerance on large grids; in fp64 the error is
+// set by the fp32 offset instead (~1e-7 grid cells).
+// First support node s = ceil(x_folded - ns/2) (may be negative) with exactly
+// the arithmetic of start_and_kernel, for a runtime width. Used by the sort to
+// bin points by their first node.
+template <typename T>
+__device__ __forceinline__ int start_index(T x, int nf, int ns, int fold64) {
+  if constexpr (sizeof(T) == 4) {
+    if (fold64) {
+      double r = fold_rescale(double(x), nf);
+      r = (r >= 0.0 && r < double(nf)) ? r : 0.0;
+      return int(std::ceil(r - ns * 0.5));
+    }
+  }
+  T r = fold_rescale(x, nf);
+  r = (r >= T(0) && r < T(nf)) ? r : T(0);
+  return int(std::ceil(r - T(ns * 0.5)));
+}
+
 template <typename T, int NS>
 __device__ __forceinline__ int start_and_kernel(T x, int nf, T *ker, const KerParams<T> &kp) {
+  if constexpr (sizeof(T) == 4) {
+    if (kp.fold64) {
+      double r = fold_rescale(double(x), nf);
+      r = (r >= 0.0 && r < double(nf)) ? r : 0.0;
+      const int s = int(std::ceil(r - NS * 0.5));
+      const double xl = double(s) - r;
+      eval_kernel<T, NS>(ker, T(xl), T(std::fma(2.0, xl, double(NS - 1))), kp);
+      return s;
+    }
+  }
   T r = fold_rescale(x, nf);
   // A non-finite coordinate folds to NaN; pin it to the grid so that every
   // index stays in bounds (its contribution is then meaningless, as it is in
   // the reference, but harmless).
   r = (r >= T(0) && r < T(nf)) ? r : T(0);
   const int s = int(std::ceil(r - T(NS * 0.5)));
-  eval_kernel<T, NS>(ker, T(s) - r, kp);
+  const T xl = T(s) - r;
+  eval_kernel<T, NS>(ker, xl, std::fma(T(2), xl, T(NS - 1)), kp);
   return s;
 }
 

@@ -34,14 +34,6 @@ void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
   }
 }
 
-template <typename T>
-size_t sm_tile_bytes(int dim, int ns, const int *binsize) {
-  const int H = (ns + 1) / 2;
-  size_t cells = 1;
-  for (int d = 0; d < dim; ++d) cells *= size_t(binsize[d] + 2 * H + 1);
-  return cells * sizeof(cplx_t<T>);
-}
-
 namespace {
 
 int device_lds_limit(int dev) {
@@ -55,6 +47,82 @@ int device_lds_limit(int dev) {
 template <typename T>
 constexpr int max_ns_of() {
   return std::is_same<T, float>::value ? MAX_NS_F32 : MAX_NS;
+}
+
+// Cells of the padded tile of a start-node bin: prod_d (b_d + ns - 1).
+int64_t tile_cells(int dim, const int *b, int ns) {
+  int64_t c = 1;
+  for (int d = 0; d < dim; ++d) c *= int64_t(b[d]) + ns - 1;
+  return c;
+}
+
+// True when whole-bin spreading can run in 2^dim parity colors with a plain
+// tile flush: per dimension an even number (>= 2) of equal bins, each at least
+// ns - 1 nodes wide (so that same-parity tiles never touch).
+bool colorable(int dim, const int *nf, const int *b, int ns) {
+  for (int d = 0; d < dim; ++d) {
+    if (b[d] < ns - 1 || nf[d] % b[d] != 0) return false;
+    const int nb = nf[d] / b[d];
+    if (nb < 2 || nb % 2 != 0) return false;
+  }
+  return true;
+}
+
+// Bin shape for the LDS kernels: the largest bin volume whose tile fits in
+// `budget` bytes (cell_bytes per tile cell) with enough bins to fill the GPU
+// (min_units work units, per color when colored). With want_colored the
+// search is over colorable shapes only; returns false when none fits.
+bool choose_bins(int dim, const int *nf, int ns, size_t cell_bytes, size_t budget,
+                 bool want_colored, int64_t min_units, int *out) {
+  std::vector<int> cand[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d >= dim) {
+      cand[d] = {1};
+      continue;
+    }
+    for (int b = 1; b <= nf[d]; ++b) {
+      if (want_colored) {
+        if (nf[d] % b || b < ns - 1 || (nf[d] / b) % 2 || nf[d] / b < 2) continue;
+      } else if (b > 8 && nf[d] % b && (b & (b - 1))) {
+        continue;  // keep the search small: divisors, powers of two and b <= 8
+      }
+      cand[d].push_back(b);
+    }
+    if (cand[d].empty()) return false;
+  }
+  bool found = false;
+  int64_t best_vol = 0;
+  double best_ratio = 1e300;
+  bool best_fill = false;
+  for (int bx : cand[0])
+    for (int by : cand[1])
+      for (int bz : cand[2]) {
+        const int b[3] = {bx, by, bz};
+        const int64_t cells = tile_cells(dim, b, ns);
+        if (size_t(cells) * cell_bytes > budget) continue;
+        int64_t vol = 1, units = 1;
+        for (int d = 0; d < dim; ++d) {
+          vol *= b[d];
+          units *= (nf[d] + b[d] - 1) / b[d];
+        }
+        if (want_colored) units >>= dim;
+        const bool fill = units >= min_units;
+        const double ratio = double(cells) / double(vol);
+        // Prefer shapes that fill the GPU, then bigger bins, then less padding.
+        const bool better = !found || (fill && !best_fill) ||
+                            (fill == best_fill && (vol > best_vol ||
+                                                   (vol == best_vol && ratio < best_ratio)));
+        if (better) {
+          found = true;
+          best_fill = fill;
+          best_vol = vol;
+          best_ratio = ratio;
+          out[0] = bx;
+          out[1] = by;
+          out[2] = bz;
+        }
+      }
+  return found;
 }
 
 }  // namespace
@@ -95,7 +163,10 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
 
   device_guard dg(opts_.gpu_device_id);
   upload_coefficients();
-  setup_bins();
+  // Method choice (makeplan.cu:308-314): LDS subproblems for types 1 and 3,
+  // point driven for type 2.
+  auto_method_ = opts_.gpu_method == 0;
+  if (auto_method_) opts_.gpu_method = (type_ == 1 || type_ == 3) ? 2 : 1;
 
   if (type_ != 3) {
     for (int d = 0; d < dim_; ++d) {
@@ -109,6 +180,7 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
     nftot_ = int64_t(nf_[0]) * nf_[1] * nf_[2];
     nmodes_ = int64_t(ms_[0]) * ms_[1] * ms_[2];
     if (nftot_ > MAX_NF) fail(FINUFFT_ERR_MAXNALLOC, "fine grid exceeds the int32 index range");
+    setup_bins();
   }
   // Vectors per FFT batch: the user's choice, else up to 8 vectors while the
   // batch grid stays within 2 GiB.
@@ -165,38 +237,47 @@ void Plan<T>::upload_coefficients() {
 
 template <typename T>
 void Plan<T>::setup_bins() {
-  // Method choice and bin sizes (makeplan.cu:308-329; heuristics.cu:325-445,
-  // with the LDS rule of this GPU).
-  const bool auto_method = opts_.gpu_method == 0;
-  if (auto_method) opts_.gpu_method = (type_ == 1 || type_ == 3) ? 2 : 1;
+  // Bin sizes per method (heuristics.cu:325-445, re-derived for this GPU's
+  // LDS). Needs nf_. Method 1 keeps the reference's bins of the folded
+  // coordinate; the LDS methods bin by first support node.
   const int lds_limit = device_lds_limit(opts_.gpu_device_id);
   int *bs = geom_.binsize;
   const int user[3] = {opts_.gpu_binsizex, opts_.gpu_binsizey, opts_.gpu_binsizez};
-  auto set_bins = [&](int b) {
+  const bool user_bins = user[0] > 0 || user[1] > 0 || user[2] > 0;
+  auto set_ref_bins = [&]() {
+    const int b = dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 8;
     for (int d = 0; d < 3; ++d) bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : b) : 1;
+    geom_.start_bins = 0;
+    colored_ = false;
+    lds_bytes_ = 0;
   };
   if (opts_.gpu_method == 1) {
-    set_bins(dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 8);
+    set_ref_bins();
   } else {
-    int b = dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 0;
-    if (b == 0) {
-      // Largest cubic bin whose padded tile fits in 128 KiB of LDS.
-      const double cells = double(std::min(lds_limit, 128 * 1024)) / sizeof(C);
-      const int H = (ns_ + 1) / 2;
-      b = int(std::floor(std::cbrt(cells) + 1e-9)) - (2 * H + 1);
+    // Spreading accumulates in fp64 LDS tiles (16 bytes per complex cell);
+    // interpolation stages the grid in the plan precision.
+    const bool spreads = type_ != 2;
+    const size_t cell = spreads ? 2 * sizeof(double) : 2 * sizeof(T);
+    bool ok = true;
+    if (user_bins) {
+      for (int d = 0; d < 3; ++d) bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : 8) : 1;
+      ok = size_t(tile_cells(dim_, bs, ns_)) * cell <= size_t(lds_limit);
+    } else {
+      const int64_t min_units = 1024;
+      ok = spreads && choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), true, min_units, bs);
+      if (!ok) ok = choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), false, min_units, bs);
     }
-    set_bins(b);
-    const bool ok = bs[0] >= 1 && bs[1] >= 1 && bs[2] >= 1 &&
-                    sm_tile_bytes<T>(dim_, ns_, bs) <= size_t(lds_limit);
     if (!ok) {
-      if (auto_method) {
+      if (auto_method_) {
         opts_.gpu_method = 1;
-        set_bins(dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 8);
+        set_ref_bins();
       } else {
         fail(FINUFFT_ERR_INSUFFICIENT_SHMEM, "LDS tile does not fit for gpu_method 2/3");
       }
     } else {
-      lds_bytes_ = sm_tile_bytes<T>(dim_, ns_, bs);
+      geom_.start_bins = 1;
+      colored_ = spreads && colorable(dim_, nf_, bs, ns_);
+      lds_bytes_ = size_t(tile_cells(dim_, bs, ns_)) * cell;
     }
   }
   for (int d = 0; d < 3; ++d)
@@ -206,15 +287,17 @@ void Plan<T>::setup_bins() {
   opts_.gpu_binsizey = bs[1];
   opts_.gpu_binsizez = bs[2];
   geom_.dim = dim_;
+  geom_.ns = ns_;
+  geom_.fold64 = std::is_same<T, float>::value ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    geom_.nf[d] = nf_[d];
+    geom_.nbins[d] = (nf_[d] + bs[d] - 1) / bs[d];
+  }
 }
 
 template <typename T>
 void Plan<T>::prepare_grid() {
-  // Bin counts, the work grid, the FFT plan and the kernel's Fourier series.
-  for (int d = 0; d < 3; ++d) {
-    geom_.nf[d] = nf_[d];
-    geom_.nbins[d] = (nf_[d] + geom_.binsize[d] - 1) / geom_.binsize[d];
-  }
+  // The work grid, the FFT plan and the kernel's Fourier series.
   if (!spread_only_ || type_ == 3) {
     fw_.resize(size_t(nftot_) * batchsize_);
   }
@@ -254,6 +337,11 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
     iota(idx_.get(), M_, stream_);
   } else {
     bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_);
+    // Coordinates in sorted order, so that spread/interp read them coalesced.
+    for (int d = 0; d < dim_; ++d) {
+      sxyz_[d].reserve_at_least(std::max(M_, 1));
+      permute<T>(idx_.get(), pts[d], sxyz_[d].get(), M_, stream_);
+    }
   }
   if (opts_.gpu_method != 1) {
     sub_capacity_ = subproblem_capacity(M_, nb, opts_.gpu_maxsubprobsize);
@@ -305,6 +393,7 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
     fail(FINUFFT_ERR_MAXNALLOC, "type 3: fine grid batch exceeds the int32 index range");
   for (int d = 0; d < 3; ++d) nf_[d] = int(nf123[d]);
   nftot_ = nft;
+  setup_bins();
 
   for (int d = 0; d < dim_; ++d) {
     xyzp_[d].reserve_at_least(std::max(M, 1));
@@ -359,15 +448,18 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.ns = ns_;
   c.method = opts_.gpu_method == 1 ? 1 : 2;
   c.M = M_;
+  const bool sorted = !(opts_.gpu_method == 1 && !opts_.gpu_sort);
   for (int d = 0; d < 3; ++d) {
     c.nf[d] = nf_[d];
     c.xyz[d] = xyz_[d];
+    c.sxyz[d] = (d < dim_ && sorted) ? sxyz_[d].get() : (d < dim_ ? xyz_[d] : nullptr);
     c.binsize[d] = geom_.binsize[d];
     c.nbins[d] = geom_.nbins[d];
   }
   c.kp.beta = T(beta_);
   c.kp.es_c = T(4.0 / (double(ns_) * ns_));
   c.kp.horner = opts_.gpu_kerevalmeth == 1 ? 1 : 0;
+  c.kp.fold64 = std::is_same<T, float>::value ? 1 : 0;
   c.kp.coef = coef_.get();
   c.idx = idx_.get();
   c.binstart = binstart_.get();
@@ -376,6 +468,7 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.sub2bin = sub2bin_.get();
   c.sub_capacity = sub_capacity_;
   c.lds_bytes = lds_bytes_;
+  c.colored = colored_ ? 1 : 0;
   return c;
 }
 
@@ -388,11 +481,19 @@ void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
     const C *c = d_c + int64_t(i) * batchsize_ * M_;
     C *fk = d_fk + int64_t(i) * batchsize_ * nmodes_;
     C *fw = spread_only_ ? fk : fw_.get();
+    timer_.begin(ST_ZERO, stream_);
     zero<T>(fw, nftot_ * blk, stream_);
+    timer_.end(stream_);
+    timer_.begin(ST_SPREAD, stream_);
     spread<T>(ctx, c, fw, blk, stream_);
+    timer_.end(stream_);
     if (spread_only_) continue;
+    timer_.begin(ST_FFT, stream_);
     fft_.exec(fw, fft_sign, stream_);
+    timer_.end(stream_);
+    timer_.begin(ST_DECONV, stream_);
     deconvolve<T>(dim_, opts_.modeord, true, ms_, nf_, ker, fw, fk, blk, stream_);
+    timer_.end(stream_);
   }
 }
 
@@ -407,10 +508,16 @@ void Plan<T>::exec_type2(C *d_c, C *d_fk, int fft_sign, int ntr) {
     C *fw = fk;
     if (!spread_only_) {
       fw = fw_.get();
+      timer_.begin(ST_DECONV, stream_);
       deconvolve<T>(dim_, opts_.modeord, false, ms_, nf_, ker, fw, fk, blk, stream_);
+      timer_.end(stream_);
+      timer_.begin(ST_FFT, stream_);
       fft_.exec(fw, fft_sign, stream_);
+      timer_.end(stream_);
     }
+    timer_.begin(ST_INTERP, stream_);
     interp<T>(ctx, c, fw, blk, stream_);
+    timer_.end(stream_);
   }
 }
 

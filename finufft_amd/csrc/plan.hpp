@@ -57,6 +57,7 @@ class Plan {
   const int *bin_start() const { return binstart_.get(); }
   int nbins_total() const { return geom_.total(); }
   hipStream_t stream() const { return stream_; }
+  StageTimer &timer() { return timer_; }
 
  private:
   void setup_kernel();
@@ -76,6 +77,7 @@ class Plan {
   int batchsize_ = 1;
   hipStream_t stream_ = nullptr;
   bool spread_only_ = false;
+  bool auto_method_ = false;
 
   // kernel
   int ns_ = 2;
@@ -97,6 +99,8 @@ class Plan {
   // sort / subproblems
   BinGeom geom_;
   dbuf<int> idx_, binstart_, substart_, sub2bin_;
+  dbuf<T> sxyz_[3];  // coordinates in sorted order
+  bool colored_ = false;
   int sub_capacity_ = 0;
   SortScratch scratch_;
   size_t lds_bytes_ = 0;
@@ -110,6 +114,8 @@ class Plan {
   dbuf<C> prephase_, deconv_, cpre_;
   double t3C_[3] = {0, 0, 0}, t3D_[3] = {0, 0, 0};
   std::unique_ptr<Plan<T>> inner_;
+
+  StageTimer timer_;
 };
 
 }  // namespace fa

@@ -196,6 +196,20 @@ __global__ void bin_key_kernel(int M, const T *__restrict__ x, const T *__restri
                                int *__restrict__ keys) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
+  if (g.start_bins) {
+    const T *c[3] = {x, y, z};
+    int key = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      int s = start_index(c[d][i], g.nf[d], g.ns, g.fold64);
+      s = s < 0 ? s + g.nf[d] : s;
+      const int b = min(max(s / g.binsize[d], 0), g.nbins[d] - 1);
+      key += b * stride;
+      stride *= g.nbins[d];
+    }
+    keys[i] = key;
+    return;
+  }
   int key = bin_of(folded(x[i], g.nf[0]), inv0, g.nbins[0]);
   if constexpr (DIM > 1) key += g.nbins[0] * bin_of(folded(y[i], g.nf[1]), inv1, g.nbins[1]);
   if constexpr (DIM > 2)
@@ -209,6 +223,13 @@ __global__ void bin_bounds_kernel(const int *sk, int M, int nb, int *binstart) {
   const int prev = i == 0 ? -1 : sk[i - 1];
   const int cur = i == M ? nb : sk[i];
   for (int b = prev + 1; b <= cur; ++b) binstart[b] = i;
+}
+
+template <typename T>
+__global__ void permute_kernel(const int *__restrict__ idx, const T *__restrict__ in,
+                               T *__restrict__ out, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) out[i] = in[idx[i]];
 }
 
 __global__ void iota_kernel(int *idx, int M) {
@@ -247,6 +268,15 @@ void exclusive_scan(const int *in, int *out, int n, SortScratch &s, hipStream_t 
   s.scan_tmp.reserve_at_least(need + 1);
   scan_rec(in, out, n, s.scan_tmp.get(), stream);
 }
+
+template <typename T>
+void permute(const int *idx, const T *in, T *out, int M, hipStream_t stream) {
+  if (M <= 0) return;
+  permute_kernel<T><<<blocks_for(M, 256), 256, 0, stream>>>(idx, in, out, M);
+  launch_check("permute_kernel");
+}
+template void permute<float>(const int *, const float *, float *, int, hipStream_t);
+template void permute<double>(const int *, const double *, double *, int, hipStream_t);
 
 void iota(int *idx, int M, hipStream_t stream) {
   if (M <= 0) return;

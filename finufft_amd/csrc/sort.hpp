@@ -20,6 +20,14 @@ struct BinGeom {
   int nf[3] = {1, 1, 1};
   int binsize[3] = {1, 1, 1};
   int nbins[3] = {1, 1, 1};
+  // 0: the reference's bins of the folded coordinate, floor(r / binsize)
+  //    (include/cufinufft/spreadinterp.hpp:214-229);
+  // 1: bins of the first support node s = ceil(r - ns/2) (wrapped into
+  //    [0, nf)), computed with the spreading kernels' own fold, so that a bin
+  //    of size b owns exactly the nodes [off, off + b + ns - 1).
+  int start_bins = 0;
+  int ns = 2;
+  int fold64 = 0;
   int total() const { return nbins[0] * nbins[1] * nbins[2]; }
 };
 
@@ -44,6 +52,10 @@ void build_subproblems(const int *binstart, int nbins, int maxsub, int *substart
 inline int subproblem_capacity(int M, int nbins, int maxsub) {
   return nbins + (M + maxsub - 1) / maxsub;
 }
+
+// out[i] = in[idx[i]] for i in [0, M).
+template <typename T>
+void permute(const int *idx, const T *in, T *out, int M, hipStream_t stream);
 
 // out[i] = sum_{k<i} in[k] for i in [0, n); in and out may alias.
 void exclusive_scan(const int *in, int *out, int n, SortScratch &s, hipStream_t stream);

@@ -48,6 +48,13 @@ __device__ __forceinline__ int wrap_once(int i, int n) { return i >= n ? i - n :
 
 // ---------------------------------------------------------------- method 1
 
+// Coordinate d of the i-th point in sorted order.
+template <typename T>
+__device__ __forceinline__ T coord(const SpreadCtx<T> &ctx, int d, int i, int j) {
+  return ctx.sxyz[d] ? ctx.sxyz[d][i] : ctx.xyz[d][j];
+}
+
+
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(SPREAD_THREADS)
     spread_gm_kernel(SpreadCtx<T> ctx, const cplx_t<T> *__restrict__ c,
@@ -60,7 +67,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS)
   int st[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    st[d] = start_and_kernel<T, NS>(ctx.xyz[d][j], ctx.nf[d], ker[d], ctx.kp);
+    st[d] = start_and_kernel<T, NS>(coord(ctx, d, i, j), ctx.nf[d], ker[d], ctx.kp);
     if (st[d] < 0) st[d] += ctx.nf[d];
   }
   const cplx_t<T> cv = c[int64_t(t) * ctx.M + j];
@@ -111,7 +118,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS)
   int st[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    st[d] = start_and_kernel<T, NS>(ctx.xyz[d][j], ctx.nf[d], ker[d], ctx.kp);
+    st[d] = start_and_kernel<T, NS>(coord(ctx, d, i, j), ctx.nf[d], ker[d], ctx.kp);
     if (st[d] < 0) st[d] += ctx.nf[d];
   }
   int xi[NS];
@@ -168,25 +175,21 @@ __global__ void __launch_bounds__(SPREAD_THREADS)
 
 // ---------------------------------------------------------------- method 2
 
-// Geometry of one subproblem: its points and its padded LDS tile. The tile
-// spans bin + ceil(ns/2) cells on the low side and ceil(ns/2) + 1 on the high
-// side; the extra high cell absorbs a point that floor(r * (1/binsize))
-// rounded into the bin below its true one.
+constexpr int BIN_THREADS = 512;
+
+// One LDS work unit of method 2: a bin (all of its points) or a subproblem
+// (<= maxsub points of one bin). Bins are bins of the first support node
+// (the sort's start_bins mode), so a bin of b nodes owns exactly the padded
+// tile [off, off + b + ns - 1) in each dimension.
 template <int DIM, int NS>
-struct SubTile {
-  static constexpr int H = (NS + 1) / 2;
+struct StartTile {
   int first, npts;
   int off[3];
   int P[3];
   int ntile;
 
   template <typename T>
-  __device__ bool init(const SpreadCtx<T> &ctx, int s) {
-    const int nb_tot = ctx.nbins[0] * ctx.nbins[1] * ctx.nbins[2];
-    if (s >= ctx.substart[nb_tot]) return false;
-    const int b = ctx.sub2bin[s];
-    first = ctx.binstart[b] + (s - ctx.substart[b]) * ctx.maxsub;
-    npts = min(ctx.maxsub, ctx.binstart[b + 1] - first);
+  __device__ void set_geometry(const SpreadCtx<T> &ctx, int b) {
     int rem = b;
     ntile = 1;
 #pragma unroll
@@ -194,13 +197,39 @@ struct SubTile {
       const int bd = rem % ctx.nbins[d];
       rem /= ctx.nbins[d];
       off[d] = bd * ctx.binsize[d];
-      P[d] = d < DIM ? ctx.binsize[d] + 2 * H + 1 : 1;
+      P[d] = d < DIM ? ctx.binsize[d] + NS - 1 : 1;
       ntile *= P[d];
     }
+  }
+
+  // Work unit = subproblem s (false when s is past the device-side count).
+  template <typename T>
+  __device__ bool init_sub(const SpreadCtx<T> &ctx, int s) {
+    const int nb_tot = ctx.nbins[0] * ctx.nbins[1] * ctx.nbins[2];
+    if (s >= ctx.substart[nb_tot]) return false;
+    const int b = ctx.sub2bin[s];
+    first = ctx.binstart[b] + (s - ctx.substart[b]) * ctx.maxsub;
+    npts = min(ctx.maxsub, ctx.binstart[b + 1] - first);
+    set_geometry(ctx, b);
     return true;
   }
 
-  // Fine-grid index of tile cell n, or -1 when no point can reach it.
+  // Work unit = whole bin b.
+  template <typename T>
+  __device__ void init_bin(const SpreadCtx<T> &ctx, int b) {
+    first = ctx.binstart[b];
+    npts = ctx.binstart[b + 1] - first;
+    set_geometry(ctx, b);
+  }
+
+  // Tile offset of a point's first node s; in [0, binsize) by construction of
+  // the sort key (the clamp only guards memory).
+  __device__ __forceinline__ int local_start(int s, int d, int nf) const {
+    const int l = (s < 0 ? s + nf : s) - off[d];
+    return min(max(l, 0), P[d] - NS);
+  }
+
+  // Fine-grid index of tile cell n.
   template <typename T>
   __device__ int64_t grid_index(const SpreadCtx<T> &ctx, int n) const {
     int64_t gi = 0, stride = 1;
@@ -208,9 +237,8 @@ struct SubTile {
     for (int d = 0; d < DIM; ++d) {
       const int nd = n % P[d];
       n /= P[d];
-      int g = nd + off[d] - H;
-      if (g >= ctx.nf[d] + H) return -1;
-      g = g < 0 ? g + ctx.nf[d] : (g >= ctx.nf[d] ? g - ctx.nf[d] : g);
+      int g = nd + off[d];
+      g = g >= ctx.nf[d] ? g - ctx.nf[d] : g;
       gi += g * stride;
       stride *= ctx.nf[d];
     }
@@ -218,97 +246,171 @@ struct SubTile {
   }
 };
 
+// Spreads the unit's points into the LDS tile held as two fp64 planes.
+// Accumulating in fp64 is what makes LDS spreading fast on this GPU: its fp64
+// LDS atomic add runs about 20x faster than the fp32 one, and the sums are
+// exact to fp32 rounding of each contribution.
 template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(SPREAD_THREADS)
-    spread_sm_kernel(SpreadCtx<T> ctx, const cplx_t<T> *__restrict__ c,
-                     cplx_t<T> *__restrict__ fw, int64_t nftot) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cplx_t<T> *tile = reinterpret_cast<cplx_t<T> *>(lds_raw);
-  SubTile<DIM, NS> st;
-  if (!st.init(ctx, blockIdx.x)) return;
-  constexpr int H = SubTile<DIM, NS>::H;
-  const int t = blockIdx.y;
-  for (int n = threadIdx.x; n < st.ntile; n += SPREAD_THREADS) tile[n] = cplx_t<T>{0, 0};
-  __syncthreads();
-  for (int i = threadIdx.x; i < st.npts; i += SPREAD_THREADS) {
-    const int j = ctx.idx[st.first + i];
+__device__ __forceinline__ void spread_into_tile(const SpreadCtx<T> &ctx,
+                                                 const StartTile<DIM, NS> &st,
+                                                 const cplx_t<T> *__restrict__ c, double *re,
+                                                 double *im) {
+  for (int i = threadIdx.x; i < st.npts; i += BIN_THREADS) {
+    const int q = st.first + i;
+    const int j = ctx.idx[q];
     T ker[DIM][NS];
     int l[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
-      l[d] = start_and_kernel<T, NS>(ctx.xyz[d][j], ctx.nf[d], ker[d], ctx.kp) - st.off[d] + H;
-    const cplx_t<T> cv = c[int64_t(t) * ctx.M + j];
+      l[d] = st.local_start(start_and_kernel<T, NS>(coord(ctx, d, q, j), ctx.nf[d], ker[d], ctx.kp),
+                            d, ctx.nf[d]);
+    const cplx_t<T> cv = c[j];
     if constexpr (DIM == 1) {
 #pragma unroll
-      for (int a = 0; a < NS; ++a) lds_add<T>(tile + l[0] + a, cv.x * ker[0][a], cv.y * ker[0][a]);
+      for (int a = 0; a < NS; ++a) {
+        atomicAdd(re + l[0] + a, double(cv.x * ker[0][a]));
+        atomicAdd(im + l[0] + a, double(cv.y * ker[0][a]));
+      }
     } else if constexpr (DIM == 2) {
       for (int b = 0; b < NS; ++b) {
         const T wr = cv.x * ker[1][b], wi = cv.y * ker[1][b];
-        cplx_t<T> *row = tile + (l[1] + b) * st.P[0] + l[0];
+        const int row = (l[1] + b) * st.P[0] + l[0];
 #pragma unroll
-        for (int a = 0; a < NS; ++a) lds_add<T>(row + a, wr * ker[0][a], wi * ker[0][a]);
+        for (int a = 0; a < NS; ++a) {
+          atomicAdd(re + row + a, double(wr * ker[0][a]));
+          atomicAdd(im + row + a, double(wi * ker[0][a]));
+        }
       }
     } else {
       for (int e = 0; e < NS; ++e) {
         const T zr = cv.x * ker[2][e], zi = cv.y * ker[2][e];
         for (int b = 0; b < NS; ++b) {
           const T wr = zr * ker[1][b], wi = zi * ker[1][b];
-          cplx_t<T> *row = tile + ((l[2] + e) * st.P[1] + l[1] + b) * st.P[0] + l[0];
+          const int row = ((l[2] + e) * st.P[1] + l[1] + b) * st.P[0] + l[0];
 #pragma unroll
-          for (int a = 0; a < NS; ++a) lds_add<T>(row + a, wr * ker[0][a], wi * ker[0][a]);
+          for (int a = 0; a < NS; ++a) {
+            atomicAdd(re + row + a, double(wr * ker[0][a]));
+            atomicAdd(im + row + a, double(wi * ker[0][a]));
+          }
         }
       }
     }
   }
+}
+
+// Colored whole-bin spreading: the launch covers the bins whose index parity
+// in every dimension equals col[]; their tiles are disjoint (the plan checks an
+// even number of equal bins of >= ns - 1 nodes per dimension), so each tile is
+// added to the grid with plain loads and stores.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(BIN_THREADS)
+    spread_bin_colored_kernel(SpreadCtx<T> ctx, const cplx_t<T> *__restrict__ c,
+                              cplx_t<T> *__restrict__ fw, int64_t nftot, int col0, int col1,
+                              int col2) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *re = reinterpret_cast<double *>(lds_raw);
+  const int col[3] = {col0, col1, col2};
+  int rem = blockIdx.x, b = 0, stride = 1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int hb = d < DIM ? (ctx.nbins[d] - col[d] + 1) / 2 : 1;
+    const int bd = d < DIM ? 2 * (rem % hb) + col[d] : 0;
+    rem /= hb;
+    b += bd * stride;
+    stride *= ctx.nbins[d];
+  }
+  StartTile<DIM, NS> st;
+  st.init_bin(ctx, b);
+  if (st.npts == 0) return;
+  double *im = re + st.ntile;
+  const int t = blockIdx.y;
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
+    re[n] = 0.0;
+    im[n] = 0.0;
+  }
+  __syncthreads();
+  spread_into_tile<T, DIM, NS>(ctx, st, c + int64_t(t) * ctx.M, re, im);
   __syncthreads();
   cplx_t<T> *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += SPREAD_THREADS) {
-    const cplx_t<T> v = tile[n];
-    if (v.x == T(0) && v.y == T(0)) continue;
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
     const int64_t gi = st.grid_index(ctx, n);
-    if (gi >= 0) hbm_add<T>(g + gi, v.x, v.y);
+    cplx_t<T> v = g[gi];
+    v.x += T(re[n]);
+    v.y += T(im[n]);
+    g[gi] = v;
   }
 }
 
+// Subproblem spreading with an atomic flush (any geometry, any density).
 template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(SPREAD_THREADS)
-    interp_sm_kernel(SpreadCtx<T> ctx, cplx_t<T> *__restrict__ c,
-                     const cplx_t<T> *__restrict__ fw, int64_t nftot) {
+__global__ void __launch_bounds__(BIN_THREADS)
+    spread_sub_kernel(SpreadCtx<T> ctx, const cplx_t<T> *__restrict__ c,
+                      cplx_t<T> *__restrict__ fw, int64_t nftot) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  cplx_t<T> *tile = reinterpret_cast<cplx_t<T> *>(lds_raw);
-  SubTile<DIM, NS> st;
-  if (!st.init(ctx, blockIdx.x)) return;
-  constexpr int H = SubTile<DIM, NS>::H;
+  double *re = reinterpret_cast<double *>(lds_raw);
+  StartTile<DIM, NS> st;
+  if (!st.init_sub(ctx, blockIdx.x)) return;
+  double *im = re + st.ntile;
   const int t = blockIdx.y;
-  const cplx_t<T> *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += SPREAD_THREADS) {
-    const int64_t gi = st.grid_index(ctx, n);
-    tile[n] = gi >= 0 ? g[gi] : cplx_t<T>{0, 0};
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
+    re[n] = 0.0;
+    im[n] = 0.0;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < st.npts; i += SPREAD_THREADS) {
-    const int j = ctx.idx[st.first + i];
+  spread_into_tile<T, DIM, NS>(ctx, st, c + int64_t(t) * ctx.M, re, im);
+  __syncthreads();
+  cplx_t<T> *g = fw + int64_t(t) * nftot;
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
+    const T vr = T(re[n]), vi = T(im[n]);
+    if (vr == T(0) && vi == T(0)) continue;
+    hbm_add<T>(g + st.grid_index(ctx, n), vr, vi);
+  }
+}
+
+// Whole-bin interpolation: stage the padded tile, then gather per point.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(BIN_THREADS)
+    interp_bin_kernel(SpreadCtx<T> ctx, cplx_t<T> *__restrict__ c,
+                      const cplx_t<T> *__restrict__ fw, int64_t nftot) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *re = reinterpret_cast<T *>(lds_raw);
+  StartTile<DIM, NS> st;
+  st.init_bin(ctx, blockIdx.x);
+  if (st.npts == 0) return;
+  T *im = re + st.ntile;
+  const int t = blockIdx.y;
+  const cplx_t<T> *g = fw + int64_t(t) * nftot;
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
+    const cplx_t<T> v = g[st.grid_index(ctx, n)];
+    re[n] = v.x;
+    im[n] = v.y;
+  }
+  __syncthreads();
+  cplx_t<T> *cout = c + int64_t(t) * ctx.M;
+  for (int i = threadIdx.x; i < st.npts; i += BIN_THREADS) {
+    const int q = st.first + i;
+    const int j = ctx.idx[q];
     T ker[DIM][NS];
     int l[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
-      l[d] = start_and_kernel<T, NS>(ctx.xyz[d][j], ctx.nf[d], ker[d], ctx.kp) - st.off[d] + H;
+      l[d] = st.local_start(start_and_kernel<T, NS>(coord(ctx, d, q, j), ctx.nf[d], ker[d], ctx.kp),
+                            d, ctx.nf[d]);
     T ar = 0, ai = 0;
     if constexpr (DIM == 1) {
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
-        const cplx_t<T> v = tile[l[0] + a];
-        ar = std::fma(v.x, ker[0][a], ar);
-        ai = std::fma(v.y, ker[0][a], ai);
+        ar = std::fma(re[l[0] + a], ker[0][a], ar);
+        ai = std::fma(im[l[0] + a], ker[0][a], ai);
       }
     } else if constexpr (DIM == 2) {
       for (int b = 0; b < NS; ++b) {
-        const cplx_t<T> *row = tile + (l[1] + b) * st.P[0] + l[0];
+        const int row = (l[1] + b) * st.P[0] + l[0];
         T rr = 0, ri = 0;
 #pragma unroll
         for (int a = 0; a < NS; ++a) {
-          rr = std::fma(row[a].x, ker[0][a], rr);
-          ri = std::fma(row[a].y, ker[0][a], ri);
+          rr = std::fma(re[row + a], ker[0][a], rr);
+          ri = std::fma(im[row + a], ker[0][a], ri);
         }
         ar = std::fma(rr, ker[1][b], ar);
         ai = std::fma(ri, ker[1][b], ai);
@@ -317,12 +419,12 @@ __global__ void __launch_bounds__(SPREAD_THREADS)
       for (int e = 0; e < NS; ++e) {
         T pr = 0, pi = 0;
         for (int b = 0; b < NS; ++b) {
-          const cplx_t<T> *row = tile + ((l[2] + e) * st.P[1] + l[1] + b) * st.P[0] + l[0];
+          const int row = ((l[2] + e) * st.P[1] + l[1] + b) * st.P[0] + l[0];
           T rr = 0, ri = 0;
 #pragma unroll
           for (int a = 0; a < NS; ++a) {
-            rr = std::fma(row[a].x, ker[0][a], rr);
-            ri = std::fma(row[a].y, ker[0][a], ri);
+            rr = std::fma(re[row + a], ker[0][a], rr);
+            ri = std::fma(im[row + a], ker[0][a], ri);
           }
           pr = std::fma(rr, ker[1][b], pr);
           pi = std::fma(ri, ker[1][b], pi);
@@ -331,7 +433,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS)
         ai = std::fma(pi, ker[2][e], ai);
       }
     }
-    c[int64_t(t) * ctx.M + j] = cplx_t<T>{ar, ai};
+    cout[j] = cplx_t<T>{ar, ai};
   }
 }
 
@@ -359,12 +461,27 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
     dim3 grid((ctx.M + SPREAD_THREADS - 1) / SPREAD_THREADS, blk);
     spread_gm_kernel<T, DIM, NS><<<grid, SPREAD_THREADS, 0, stream>>>(ctx, c, fw, nftot);
     launch_check("spread_gm_kernel");
+  } else if (ctx.colored) {
+    auto k = spread_bin_colored_kernel<T, DIM, NS>;
+    allow_lds(k, ctx.lds_bytes);
+    for (int color = 0; color < (1 << DIM); ++color) {
+      int col[3] = {0, 0, 0};
+      int64_t nblk = 1;
+      for (int d = 0; d < DIM; ++d) {
+        col[d] = (color >> d) & 1;
+        nblk *= (ctx.nbins[d] - col[d] + 1) / 2;
+      }
+      if (nblk == 0) continue;
+      k<<<dim3(unsigned(nblk), blk), BIN_THREADS, ctx.lds_bytes, stream>>>(
+          ctx, c, fw, nftot, col[0], col[1], col[2]);
+      launch_check("spread_bin_colored_kernel");
+    }
   } else {
-    auto k = spread_sm_kernel<T, DIM, NS>;
+    auto k = spread_sub_kernel<T, DIM, NS>;
     allow_lds(k, ctx.lds_bytes);
     dim3 grid(ctx.sub_capacity, blk);
-    k<<<grid, SPREAD_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
-    launch_check("spread_sm_kernel");
+    k<<<grid, BIN_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
+    launch_check("spread_sub_kernel");
   }
 }
 
@@ -378,11 +495,11 @@ void interp_ns(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int b
     interp_gm_kernel<T, DIM, NS><<<grid, SPREAD_THREADS, 0, stream>>>(ctx, c, fw, nftot, blk);
     launch_check("interp_gm_kernel");
   } else {
-    auto k = interp_sm_kernel<T, DIM, NS>;
+    auto k = interp_bin_kernel<T, DIM, NS>;
     allow_lds(k, ctx.lds_bytes);
-    dim3 grid(ctx.sub_capacity, blk);
-    k<<<grid, SPREAD_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
-    launch_check("interp_sm_kernel");
+    dim3 grid(ctx.nbins[0] * ctx.nbins[1] * ctx.nbins[2], blk);
+    k<<<grid, BIN_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
+    launch_check("interp_bin_kernel");
   }
 }
 

@@ -32,6 +32,9 @@ struct SpreadCtx {
   int M = 0;
   int nf[3] = {1, 1, 1};
   const T *xyz[3] = {nullptr, nullptr, nullptr};
+  // Coordinates permuted into the sorted order (sxyz[d][i] = xyz[d][idx[i]]);
+  // null when not materialized.
+  const T *sxyz[3] = {nullptr, nullptr, nullptr};
   KerParams<T> kp{};
 
   const int *idx = nullptr;  // point order (sorted or identity), length M
@@ -48,6 +51,10 @@ struct SpreadCtx {
   const int *sub2bin = nullptr;
   int sub_capacity = 0;  // launch grid (>= the device-side total)
   size_t lds_bytes = 0;  // LDS tile bytes of the method-2 kernels
+  // Method-2 spreading by whole bins in 2^dim parity colors with a plain
+  // (non-atomic) tile flush; requires per dimension an even number (>= 2) of
+  // equal bins of at least 2 ceil(ns/2) + 1 cells.
+  int colored = 0;
 };
 
 // c (M x blk, contiguous per vector) -> fw (nf_tot x blk). fw must be zeroed
@@ -69,9 +76,5 @@ template <typename T, int DIM>
 void interp_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
                 hipStream_t stream);
 
-// LDS bytes the method-2 tile of this geometry needs:
-// prod_d (binsize_d + 2 ceil(ns/2) + 1) complex values.
-template <typename T>
-size_t sm_tile_bytes(int dim, int ns, const int *binsize);
 
 }  // namespace fa

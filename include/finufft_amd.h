@@ -38,6 +38,15 @@ FINUFFT_EXPORT int finufft_amd_plan_info(const void *plan, int is_double, int64_
 FINUFFT_EXPORT int finufft_amd_copy_sort(const void *plan, int is_double, int *perm,
                                          int *binstart);
 
+/* Per-stage GPU timing of execute (HIP events on the plan's stream); no
+ * reference counterpart (the reference prints CPU-side timings under
+ * opts.debug). set: on != 0 enables recording. get: waits for the recorded
+ * events, writes the milliseconds spent since the last get in ms[8] as
+ * {zero, spread, fft, deconvolve/amplify, interp, type-3 phases, 0, 0}, and
+ * resets. */
+FINUFFT_EXPORT int finufft_amd_set_stage_timing(void *plan, int is_double, int on);
+FINUFFT_EXPORT int finufft_amd_get_stage_times(void *plan, int is_double, double *ms);
+
 /* ---- plan-time math, exposed for tests and tools (host only, no GPU) ---- */
 
 /* Smallest 2,3,5-smooth multiple of f that is >= n; reference:

@@ -38,9 +38,31 @@ def bin_keys(pts_c, nf, binsize):
     return key
 
 
-def bin_sort(pts_c, nf, binsize):
-    """(permutation, binstart) of the stable bin sort."""
-    key = bin_keys(pts_c, nf, binsize)
+def start_bin_keys(pts_c, nf, binsize, ns, fold64=True):
+    """Bins of the first support node s = ceil(r - ns/2), wrapped into [0, nf),
+    with r folded in fp64 (fp32 plans) or in the points' own precision."""
+    nb = nbins(nf, binsize)
+    key = np.zeros(len(pts_c[0]), np.int64)
+    stride = 1
+    for d, x in enumerate(pts_c):
+        xx = x.astype(np.float64) if fold64 else x
+        r = cfold.fold_rescale(xx, nf[d])
+        r = np.where((r >= 0) & (r < nf[d]), r, 0)
+        s = np.ceil(r - np.asarray(ns * 0.5, r.dtype)).astype(np.int64)
+        s = np.where(s < 0, s + nf[d], s)
+        b = np.clip(s // binsize[d], 0, nb[d] - 1)
+        key += b * stride
+        stride *= nb[d]
+    return key
+
+
+def bin_sort(pts_c, nf, binsize, start_ns=None, fold64=True):
+    """(permutation, binstart) of the stable bin sort; start_ns selects bins of
+    the first support node for kernel width start_ns."""
+    if start_ns is None:
+        key = bin_keys(pts_c, nf, binsize)
+    else:
+        key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64)
     perm = np.argsort(key, kind="stable").astype(np.int32)
     total = int(np.prod(nbins(nf, binsize)))
     counts = np.bincount(key, minlength=total)
@@ -49,7 +71,11 @@ def bin_sort(pts_c, nf, binsize):
 
 
 def bin_sort_bruteforce(pts_c, nf, binsize):
-    """Same permutation via lexsort on (index, key); for tiny inputs."""
+    """Same permutation by a plain counting loop over bins; for tiny inputs."""
     key = bin_keys(pts_c, nf, binsize)
-    idx = np.arange(len(key))
-    return np.lexsort((idx, key)).astype(np.int32)
+    out = []
+    for b in range(int(np.prod(nbins(nf, binsize)))):
+        for i in range(len(key)):
+            if key[i] == b:
+                out.append(i)
+    return np.array(out, np.int32)
