@@ -102,6 +102,9 @@ def main(argv=None):
     ap.add_argument("--err-samples", type=int, default=256)
     ap.add_argument("--method1", type=int, default=0, help="gpu_method of the type-1 plan (0 auto)")
     ap.add_argument("--method2", type=int, default=0, help="gpu_method of the type-2 plan (0 auto)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy a tensor of this many MiB (a plain copy kernel of "
+                         "known size, the reference point for HBM counter readings)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -195,6 +198,16 @@ def main(argv=None):
     ref2 = acc.type2_at(pts, f, -1, pidx)
     e2 = acc.relerr(c_out[pidx.to(dev)], ref2)
     log(f"relerr type1 {e1:.3e} type2 {e2:.3e}")
+
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
 
     cpu = None
     if not args.no_cpu:

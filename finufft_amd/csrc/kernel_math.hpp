@@ -43,11 +43,14 @@ FA_HD double next_down(double s) {
 }
 
 // a - b rounded toward -inf. TwoSum gives the exact error e of s = RN(a - b)
-// (a - b = s + e exactly); when e < 0 the exact value lies below s.
+// (a - b = s + e exactly); when e < 0 the exact value lies below s. An exact
+// zero difference is -0 when rounding toward -inf (IEEE 754 6.3), except
+// +0 - (-0) = +0.
 template <typename T>
 FA_HD T sub_rd(T a, T b) {
   const T nb = -b;
   const T s = a + nb;
+  if (s == T(0)) return (a == T(0) && !std::signbit(a) && std::signbit(b)) ? T(0) : T(-0.0);
   const T bb = s - a;
   const T e = (a - (s - bb)) + (nb - bb);
   return (e < T(0)) ? next_down(s) : s;

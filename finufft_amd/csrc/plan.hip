@@ -163,10 +163,13 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
 
   device_guard dg(opts_.gpu_device_id);
   upload_coefficients();
-  // Method choice (makeplan.cu:308-314): LDS subproblems for types 1 and 3,
-  // point driven for type 2.
+  // Method choice. The reference picks LDS subproblems for types 1 and 3 and
+  // point-driven interpolation for type 2 (makeplan.cu:308-314); on this GPU
+  // staging the grid in LDS is faster for interpolation too, so the LDS
+  // method is the default for every type, with the point-driven method as
+  // the fallback when the tile does not fit.
   auto_method_ = opts_.gpu_method == 0;
-  if (auto_method_) opts_.gpu_method = (type_ == 1 || type_ == 3) ? 2 : 1;
+  if (auto_method_) opts_.gpu_method = 2;
 
   if (type_ != 3) {
     for (int d = 0; d < dim_; ++d) {
