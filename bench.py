@@ -102,6 +102,10 @@ def main(argv=None):
     ap.add_argument("--err-samples", type=int, default=256)
     ap.add_argument("--method1", type=int, default=0, help="gpu_method of the type-1 plan (0 auto)")
     ap.add_argument("--method2", type=int, default=0, help="gpu_method of the type-2 plan (0 auto)")
+    ap.add_argument("--binsize1", type=str, default="",
+                    help="gpu_binsize x,y,z of the type-1 plan (default: the plan's choice)")
+    ap.add_argument("--binsize2", type=str, default="",
+                    help="gpu_binsize x,y,z of the type-2 plan (default: the plan's choice)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a tensor of this many MiB (a plain copy kernel of "
                          "known size, the reference point for HBM counter readings)")
@@ -136,8 +140,16 @@ def main(argv=None):
     torch.cuda.synchronize()
     log(f"inputs ready: M={M:.3g} N={N}^3")
 
-    p1 = Plan(1, shape, eps=tol, dtype="complex64", isign=1, gpu_method=args.method1)
-    p2 = Plan(2, shape, eps=tol, dtype="complex64", isign=-1, gpu_method=args.method2)
+    def bins(spec):
+        if not spec:
+            return {}
+        b = [int(v) for v in spec.split(",")]
+        return {"gpu_binsizex": b[0], "gpu_binsizey": b[1], "gpu_binsizez": b[2]}
+
+    p1 = Plan(1, shape, eps=tol, dtype="complex64", isign=1, gpu_method=args.method1,
+              **bins(args.binsize1))
+    p2 = Plan(2, shape, eps=tol, dtype="complex64", isign=-1, gpu_method=args.method2,
+              **bins(args.binsize2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     p1.setpts(*pts)

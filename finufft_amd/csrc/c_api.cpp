@@ -173,7 +173,8 @@ int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout, double
         const int64_t v[16] = {p->type(),       p->dim(),       p->ns(),     p->nf(0),
                                p->nf(1),        p->nf(2),       p->method(), p->binsize(0),
                                p->binsize(1),   p->binsize(2),  p->batchsize(), p->M(),
-                               p->nbins_total(), 0,             0,           0};
+                               p->nbins_total(), p->col_side(), p->colored(),
+                               p->dense_subproblems()};
         for (int i = 0; i < 16; ++i) iout[i] = v[i];
       }
       if (dout) {

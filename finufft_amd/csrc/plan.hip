@@ -125,6 +125,33 @@ bool choose_bins(int dim, const int *nf, int ns, size_t cell_bytes, size_t budge
   return found;
 }
 
+// Bin shape of the column-window spreader (3-D, fp32, ns <= 8): 8 x 8 first
+// nodes in (x, y) (a multiple of every sub-column side) and the tallest z
+// extent whose tile plus the staging area fits, in order of preference:
+// colorable within `budget` (two workgroups per CU), colorable within the
+// device limit, any shape within `budget`, any within the limit. Returns
+// false when nothing fits.
+bool choose_col_bins(const int *nf, int ns, size_t budget, size_t limit, int *out,
+                     bool *colored) {
+  static const int zc[] = {32, 24, 16, 12, 8, 6, 4, 2, 1};
+  for (int pass = 0; pass < 4; ++pass) {
+    const bool want_col = pass < 2;
+    const int64_t cap = int64_t(pass % 2 == 0 ? budget : limit);
+    for (int bz : zc) {
+      const int b[3] = {8, 8, bz};
+      if (tile_cells(3, b, ns) * 16 + int64_t(col_stage_bytes()) > cap) continue;
+      const bool colr = colorable(3, nf, b, ns);
+      if (want_col && !colr) continue;
+      out[0] = 8;
+      out[1] = 8;
+      out[2] = bz;
+      *colored = colr;
+      return true;
+    }
+  }
+  return false;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ makeplan
@@ -254,7 +281,34 @@ void Plan<T>::setup_bins() {
     colored_ = false;
     lds_bytes_ = 0;
   };
-  if (opts_.gpu_method == 1) {
+  geom_.colS = 0;
+  // Column-window spreading: 3-D fp32 spreading plans with ns <= 8. User bin
+  // sizes select it when x and y are 8 (the footprint of one wave).
+  const bool col_ok = opts_.gpu_method != 1 && type_ != 2 && dim_ == 3 &&
+                      std::is_same<T, float>::value && ns_ <= COL_MAX_NS &&
+                      (!user_bins || (user[0] == 8 && user[1] == 8 && user[2] > 0));
+  bool col_done = false;
+  if (col_ok) {
+    const size_t budget = std::min<size_t>(size_t(lds_limit), 80 * 1024);
+    bool colr = false;
+    if (user_bins) {
+      bs[0] = bs[1] = 8;
+      bs[2] = user[2];
+      colr = colorable(3, nf_, bs, ns_);
+      col_done = tile_cells(3, bs, ns_) * 16 + int64_t(col_stage_bytes()) <= lds_limit;
+    } else {
+      col_done = choose_col_bins(nf_, ns_, budget, size_t(lds_limit), bs, &colr);
+    }
+    if (col_done) {
+      geom_.start_bins = 1;
+      geom_.colS = fa::col_side(ns_);
+      colored_ = colr;
+      lds_bytes_ = size_t(tile_cells(3, bs, ns_)) * 16 + col_stage_bytes();
+    }
+  }
+  if (col_done) {
+    // bins chosen above
+  } else if (opts_.gpu_method == 1) {
     set_ref_bins();
   } else {
     // Spreading accumulates in fp64 LDS tiles (16 bytes per complex cell);
@@ -336,10 +390,12 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
   idx_.reserve_at_least(std::max(M_, 1));
   binstart_.reserve_at_least(size_t(nb) + 1);
   const T *pts[3] = {xyz_[0], xyz_[1], xyz_[2]};
+  if (geom_.colS) colstart_.reserve_at_least(size_t(nb) * geom_.nsub() + 1);
   if (opts_.gpu_method == 1 && !opts_.gpu_sort) {
     iota(idx_.get(), M_, stream_);
   } else {
-    bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_);
+    bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_,
+                geom_.colS ? colstart_.get() : nullptr);
     // Coordinates in sorted order, so that spread/interp read them coalesced.
     for (int d = 0; d < dim_; ++d) {
       sxyz_[d].reserve_at_least(std::max(M_, 1));
@@ -350,8 +406,19 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
     sub_capacity_ = subproblem_capacity(M_, nb, opts_.gpu_maxsubprobsize);
     substart_.reserve_at_least(size_t(nb) + 1);
     sub2bin_.reserve_at_least(std::max(sub_capacity_, 1));
-    build_subproblems(binstart_.get(), nb, opts_.gpu_maxsubprobsize, substart_.get(),
-                      sub2bin_.get(), scratch_, stream_);
+    // Column spreading runs whole bins in parity colors and only the bins
+    // denser than dense_min_ (all bins when the shape is not colorable) as
+    // subproblems; their count is read back so that the launch is exact.
+    dense_min_ = geom_.colS && colored_ ? 8 * opts_.gpu_maxsubprobsize : 0;
+    build_subproblems(binstart_.get(), nb, opts_.gpu_maxsubprobsize, dense_min_,
+                      substart_.get(), sub2bin_.get(), scratch_, stream_);
+    n_dense_sub_ = 0;
+    if (geom_.colS) {
+      hip_check(hipMemcpyAsync(&n_dense_sub_, substart_.get() + nb, sizeof(int),
+                               hipMemcpyDeviceToHost, stream_),
+                "hipMemcpyAsync(subproblem count)");
+      hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize(subproblem count)");
+    }
   }
 }
 
@@ -472,6 +539,10 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.sub_capacity = sub_capacity_;
   c.lds_bytes = lds_bytes_;
   c.colored = colored_ ? 1 : 0;
+  c.colS = geom_.colS;
+  c.colstart = colstart_.get();
+  c.dense_min = dense_min_;
+  c.n_dense_sub = n_dense_sub_;
   return c;
 }
 

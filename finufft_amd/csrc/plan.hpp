@@ -56,6 +56,9 @@ class Plan {
   const int *sort_perm() const { return idx_.get(); }
   const int *bin_start() const { return binstart_.get(); }
   int nbins_total() const { return geom_.total(); }
+  int col_side() const { return geom_.colS; }
+  int colored() const { return colored_ ? 1 : 0; }
+  int dense_subproblems() const { return n_dense_sub_; }
   hipStream_t stream() const { return stream_; }
   StageTimer &timer() { return timer_; }
 
@@ -98,10 +101,12 @@ class Plan {
 
   // sort / subproblems
   BinGeom geom_;
-  dbuf<int> idx_, binstart_, substart_, sub2bin_;
+  dbuf<int> idx_, binstart_, substart_, sub2bin_, colstart_;
   dbuf<T> sxyz_[3];  // coordinates in sorted order
   bool colored_ = false;
   int sub_capacity_ = 0;
+  int dense_min_ = 0;    // column spreading: bins above this count use subproblems
+  int n_dense_sub_ = 0;  // column spreading: number of those subproblems
   SortScratch scratch_;
   size_t lds_bytes_ = 0;
 

@@ -1,7 +1,7 @@
 // Deterministic bin sort (see sort.hpp).
 //
 // Layout of the work: the bin key of every point is computed once (4 bytes
-// per point); then 1..3 passes of a stable LSD radix sort of (key, index)
+// per point); then 1..4 passes of a stable LSD radix sort of (key, index)
 // pairs with up to 8-bit digits. Each pass is two kernels over 4096-point
 // tiles (256 lanes x 16 points) plus a scan of the digit-major tile
 // histogram:
@@ -198,14 +198,20 @@ __global__ void bin_key_kernel(int M, const T *__restrict__ x, const T *__restri
   if (i >= M) return;
   if (g.start_bins) {
     const T *c[3] = {x, y, z};
-    int key = 0, stride = 1;
+    int key = 0, stride = 1, l[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       int s = start_index(c[d][i], g.nf[d], g.ns, g.fold64);
       s = s < 0 ? s + g.nf[d] : s;
       const int b = min(max(s / g.binsize[d], 0), g.nbins[d] - 1);
+      l[d] = min(max(s - b * g.binsize[d], 0), g.binsize[d] - 1);
       key += b * stride;
       stride *= g.nbins[d];
+    }
+    if (DIM == 3 && g.colS > 0) {
+      const int nsx = g.binsize[0] / g.colS, nsy = g.binsize[1] / g.colS;
+      const int sub = min(l[0] / g.colS, nsx - 1) + nsx * min(l[1] / g.colS, nsy - 1);
+      key = (key * (nsx * nsy) + sub) * g.binsize[2] + l[2];
     }
     keys[i] = key;
     return;
@@ -217,12 +223,18 @@ __global__ void bin_key_kernel(int M, const T *__restrict__ x, const T *__restri
   keys[i] = key;
 }
 
-__global__ void bin_bounds_kernel(const int *sk, int M, int nb, int *binstart) {
+// binstart[g] = first sorted position whose key / div is >= g, g in [0, ng].
+__global__ void bin_bounds_kernel(const int *sk, int M, int ng, int div, int *binstart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > M) return;
-  const int prev = i == 0 ? -1 : sk[i - 1];
-  const int cur = i == M ? nb : sk[i];
+  const int prev = i == 0 ? -1 : sk[i - 1] / div;
+  const int cur = i == M ? ng : sk[i] / div;
   for (int b = prev + 1; b <= cur; ++b) binstart[b] = i;
+}
+
+__global__ void strided_copy_kernel(const int *in, int stride, int n, int *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) out[i] = in[int64_t(i) * stride];
 }
 
 template <typename T>
@@ -242,9 +254,13 @@ __global__ void two_point_bounds_kernel(int *binstart, int nb, int M) {
   if (b <= nb) binstart[b] = (b == nb) ? M : 0;
 }
 
-__global__ void numsub_kernel(const int *binstart, int nb, int maxsub, int *numsub) {
+__global__ void numsub_kernel(const int *binstart, int nb, int maxsub, int dense_min,
+                              int *numsub) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) numsub[b] = (binstart[b + 1] - binstart[b] + maxsub - 1) / maxsub;
+  if (b < nb) {
+    const int cnt = binstart[b + 1] - binstart[b];
+    numsub[b] = cnt > dense_min ? (cnt + maxsub - 1) / maxsub : 0;
+  }
   if (b == nb) numsub[b] = 0;
 }
 
@@ -286,12 +302,21 @@ void iota(int *idx, int M, hipStream_t stream) {
 
 template <typename T>
 void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *binstart,
-              SortScratch &s, hipStream_t stream) {
+              SortScratch &s, hipStream_t stream, int *colstart) {
   const int nb = g.total();
-  if (M == 0 || nb == 1) {
+  const bool cols = g.colS > 0 && g.dim == 3 && g.start_bins;
+  if (cols && (colstart == nullptr || g.nkeys() > INT32_MAX))
+    fail(FINUFFT_ERR_INVALID_ARGUMENT, "bin_sort: invalid sub-column geometry");
+  const int nsub = cols ? g.nsub() : 1;
+  if (M == 0 || (nb == 1 && !cols)) {
     iota(idx, M, stream);
     two_point_bounds_kernel<<<blocks_for(nb + 1, 256), 256, 0, stream>>>(binstart, nb, M);
     launch_check("two_point_bounds_kernel");
+    if (cols) {
+      two_point_bounds_kernel<<<blocks_for(int64_t(nb) * nsub + 1, 256), 256, 0, stream>>>(
+          colstart, nb * nsub, M);
+      launch_check("two_point_bounds_kernel");
+    }
     return;
   }
   s.keys_a.reserve_at_least(M);
@@ -314,7 +339,8 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
   }
   launch_check("bin_key_kernel");
 
-  const int B = 32 - __builtin_clz(unsigned(nb - 1));  // bits of the largest key
+  const int64_t nkeys = cols ? g.nkeys() : nb;
+  const int B = std::max(1, 32 - __builtin_clz(unsigned(nkeys - 1)));  // bits of the largest key
   const int passes = (B + 7) / 8;
   const int bpp = (B + passes - 1) / passes;
   const int ntiles = blocks_for(M, RS_TILE);
@@ -336,19 +362,29 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
     kin = kout;
     vin = vout;
   }
-  bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(kin, M, nb, binstart);
-  launch_check("bin_bounds_kernel");
+  if (cols) {
+    bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(
+        kin, M, nb * nsub, g.binsize[2], colstart);
+    launch_check("bin_bounds_kernel");
+    strided_copy_kernel<<<blocks_for(int64_t(nb) + 1, 256), 256, 0, stream>>>(colstart, nsub, nb,
+                                                                               binstart);
+    launch_check("strided_copy_kernel");
+  } else {
+    bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(kin, M, nb, 1,
+                                                                          binstart);
+    launch_check("bin_bounds_kernel");
+  }
 }
 
 template void bin_sort<float>(const BinGeom &, int, const float *const[3], int *, int *,
-                              SortScratch &, hipStream_t);
+                              SortScratch &, hipStream_t, int *);
 template void bin_sort<double>(const BinGeom &, int, const double *const[3], int *, int *,
-                               SortScratch &, hipStream_t);
+                               SortScratch &, hipStream_t, int *);
 
-void build_subproblems(const int *binstart, int nb, int maxsub, int *substart, int *sub2bin,
-                       SortScratch &s, hipStream_t stream) {
+void build_subproblems(const int *binstart, int nb, int maxsub, int dense_min, int *substart,
+                       int *sub2bin, SortScratch &s, hipStream_t stream) {
   numsub_kernel<<<blocks_for(int64_t(nb) + 1, 256), 256, 0, stream>>>(binstart, nb, maxsub,
-                                                                       substart);
+                                                                       dense_min, substart);
   launch_check("numsub_kernel");
   exclusive_scan(substart, substart, nb + 1, s, stream);
   sub2bin_kernel<<<blocks_for(nb, 256), 256, 0, stream>>>(substart, nb, sub2bin);

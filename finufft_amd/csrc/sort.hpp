@@ -28,7 +28,14 @@ struct BinGeom {
   int start_bins = 0;
   int ns = 2;
   int fold64 = 0;
+  // colS > 0 (3-D start bins only) refines the key inside each bin to
+  // (sub-column, local z of the first node): a sub-column is colS x colS
+  // first-node positions in x and y over the bin's whole z extent. Points of a
+  // sub-column are then contiguous and ordered by z (ties by index).
+  int colS = 0;
   int total() const { return nbins[0] * nbins[1] * nbins[2]; }
+  int nsub() const { return colS ? (binsize[0] / colS) * (binsize[1] / colS) : 1; }
+  int64_t nkeys() const { return int64_t(total()) * nsub() * (colS ? binsize[2] : 1); }
 };
 
 // Scratch reused across setpts calls (grows only).
@@ -37,17 +44,20 @@ struct SortScratch {
 };
 
 // idx[M]: sorted point order. binstart[nbins+1]: first sorted position of each
-// bin, binstart[nbins] = M. Nothing is synchronized with the host.
+// bin, binstart[nbins] = M. With g.colS, colstart[nbins * nsub + 1] receives
+// the first sorted position of each (bin, sub-column). Nothing is
+// synchronized with the host.
 template <typename T>
 void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *binstart,
-              SortScratch &s, hipStream_t stream);
+              SortScratch &s, hipStream_t stream, int *colstart = nullptr);
 
 // idx[i] = i.
 void iota(int *idx, int M, hipStream_t stream);
 
-// substart[nbins+1] (exclusive scan of ceil(count/maxsub), last entry = total)
-// and sub2bin[capacity], capacity = subproblem_capacity(M, nbins, maxsub).
-void build_subproblems(const int *binstart, int nbins, int maxsub, int *substart,
+// substart[nbins+1] (exclusive scan of ceil(count/maxsub) over the bins with
+// more than dense_min points, 0 for the others; last entry = total) and
+// sub2bin[capacity], capacity = subproblem_capacity(M, nbins, maxsub).
+void build_subproblems(const int *binstart, int nbins, int maxsub, int dense_min, int *substart,
                        int *sub2bin, SortScratch &s, hipStream_t stream);
 inline int subproblem_capacity(int M, int nbins, int maxsub) {
   return nbins + (M + maxsub - 1) / maxsub;

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -184,12 +185,14 @@ constexpr int BIN_THREADS = 512;
 template <int DIM, int NS>
 struct StartTile {
   int first, npts;
+  int bin;
   int off[3];
   int P[3];
   int ntile;
 
   template <typename T>
   __device__ void set_geometry(const SpreadCtx<T> &ctx, int b) {
+    bin = b;
     int rem = b;
     ntile = 1;
 #pragma unroll
@@ -298,6 +301,23 @@ __device__ __forceinline__ void spread_into_tile(const SpreadCtx<T> &ctx,
   }
 }
 
+// Bin of the blk-th workgroup of the launch of parity color (col0, col1, col2).
+template <int DIM, typename T>
+__device__ __forceinline__ int colored_bin(const SpreadCtx<T> &ctx, int blk, int col0, int col1,
+                                           int col2) {
+  const int col[3] = {col0, col1, col2};
+  int rem = blk, b = 0, stride = 1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int hb = d < DIM ? (ctx.nbins[d] - col[d] + 1) / 2 : 1;
+    const int bd = d < DIM ? 2 * (rem % hb) + col[d] : 0;
+    rem /= hb;
+    b += bd * stride;
+    stride *= ctx.nbins[d];
+  }
+  return b;
+}
+
 // Colored whole-bin spreading: the launch covers the bins whose index parity
 // in every dimension equals col[]; their tiles are disjoint (the plan checks an
 // even number of equal bins of >= ns - 1 nodes per dimension), so each tile is
@@ -309,18 +329,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
                               int col2) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double *re = reinterpret_cast<double *>(lds_raw);
-  const int col[3] = {col0, col1, col2};
-  int rem = blockIdx.x, b = 0, stride = 1;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int hb = d < DIM ? (ctx.nbins[d] - col[d] + 1) / 2 : 1;
-    const int bd = d < DIM ? 2 * (rem % hb) + col[d] : 0;
-    rem /= hb;
-    b += bd * stride;
-    stride *= ctx.nbins[d];
-  }
   StartTile<DIM, NS> st;
-  st.init_bin(ctx, b);
+  st.init_bin(ctx, colored_bin<DIM>(ctx, blockIdx.x, col0, col1, col2));
   if (st.npts == 0) return;
   double *im = re + st.ntile;
   const int t = blockIdx.y;
@@ -364,6 +374,270 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const T vr = T(re[n]), vi = T(im[n]);
     if (vr == T(0) && vi == T(0)) continue;
     hbm_add<T>(g + st.grid_index(ctx, n), vr, vi);
+  }
+}
+
+
+// ------------------------------------------------------- column-window (3-D)
+//
+// The per-point LDS scatter above costs ns^3 LDS atomics per point. Here a
+// wave owns the 8 x 8 (x, y) grid columns of one sub-column (the points whose
+// first node lies in an S x S square of the bin, S = col_side(ns)), one
+// column per lane, and walks the sub-column's points in increasing z of their
+// first node. Each lane keeps the ns z-planes of its column that the current
+// point can touch in registers (register k holds plane zb + k), adds
+// c * wx * wy * wz[k] to them, and hands plane zb to the LDS tile (one fp64
+// atomic add per lane) only when the walk moves past it, shifting the window
+// up one plane. Per point the wave does 2 ns FMAs per lane and reads its
+// staged kernel values from LDS; the LDS atomics drop from ns^3 per point to
+// about 64 per plane of a sub-column.
+//
+// Staging: lane k of the wave evaluates the kernel of point p + k and writes
+// one record of COL_REC floats: wx[8] and wy[8] (the x and y weights already
+// shifted to the footprint column of each lane, zero outside), wz[8], c; the
+// point's first z plane stays in that lane's register and is read with
+// readlane.
+
+// Hands plane zb (register 0) to the tile cell and shifts the window up.
+template <int NS>
+__device__ __forceinline__ void col_shift(float (&ar)[NS], float (&ai)[NS], double *re,
+                                          double *im, int cell, bool on) {
+  if (on && (ar[0] != 0.f || ai[0] != 0.f)) {
+    atomicAdd(re + cell, double(ar[0]));
+    atomicAdd(im + cell, double(ai[0]));
+  }
+#pragma unroll
+  for (int k = 0; k + 1 < NS; ++k) {
+    ar[k] = ar[k + 1];
+    ai[k] = ai[k + 1];
+  }
+  ar[NS - 1] = 0.f;
+  ai[NS - 1] = 0.f;
+}
+
+// Hands all ns window planes zb .. zb + ns - 1 to the tile and clears them.
+template <int NS>
+__device__ __forceinline__ void col_flush_all(int zb, float (&ar)[NS], float (&ai)[NS],
+                                              double *re, double *im, int colcell, int plane,
+                                              bool on) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (on && (ar[k] != 0.f || ai[k] != 0.f)) {
+      const int cell = colcell + (zb + k) * plane;
+      atomicAdd(re + cell, double(ar[k]));
+      atomicAdd(im + cell, double(ai[k]));
+    }
+    ar[k] = 0.f;
+    ai[k] = 0.f;
+  }
+}
+
+// Staged values of one point as one lane of the wave sees them.
+template <int NS>
+struct ColPoint {
+  float wx, wy, cr, ci, wz[NS];
+  __device__ __forceinline__ void load(const float *stage, int k, int li, int lj) {
+    const float *r = stage + k * COL_REC;
+    wx = r[li];
+    wy = r[8 + lj];
+    cr = r[24];
+    ci = r[25];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) wz[m] = r[16 + m];
+  }
+};
+
+// Moves the window up to plane lz (handing the passed planes to the tile) and
+// adds the point's contributions to the window.
+template <int NS>
+__device__ __forceinline__ void col_accumulate(const ColPoint<NS> &pt, int lz, int &zb,
+                                               float (&ar)[NS], float (&ai)[NS], double *re,
+                                               double *im, int colcell, bool on) {
+  constexpr int PLANE = (8 + NS - 1) * (8 + NS - 1);
+  if (lz > zb) {
+    if (lz - zb >= NS) {
+      col_flush_all<NS>(zb, ar, ai, re, im, colcell, PLANE, on);
+      zb = lz;
+    } else {
+      do {
+        col_shift<NS>(ar, ai, re, im, colcell + zb * PLANE, on);
+        ++zb;
+      } while (zb < lz);
+    }
+  }
+  const float w = pt.wx * pt.wy;
+  const float wr = w * pt.cr, wi = w * pt.ci;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+    ar[m] = std::fma(wr, pt.wz[m], ar[m]);
+    ai[m] = std::fma(wi, pt.wz[m], ai[m]);
+  }
+}
+
+// Tile side in x and y of the column spreader (bins of 8 first nodes).
+template <int NS>
+constexpr int col_tile_xy() {
+  return 8 + NS - 1;
+}
+
+// Fine-grid index of column-tile cell n (x fastest; compile-time x, y sides).
+template <int NS>
+__device__ __forceinline__ int64_t col_grid_index(const SpreadCtx<float> &ctx,
+                                                  const StartTile<3, NS> &st, int n) {
+  constexpr int P = col_tile_xy<NS>();
+  const int x = n % P, t = n / P;
+  const int y = t % P, z = t / P;
+  int gx = x + st.off[0], gy = y + st.off[1], gz = z + st.off[2];
+  gx = gx >= ctx.nf[0] ? gx - ctx.nf[0] : gx;
+  gy = gy >= ctx.nf[1] ? gy - ctx.nf[1] : gy;
+  gz = gz >= ctx.nf[2] ? gz - ctx.nf[2] : gz;
+  return (int64_t(gz) * ctx.nf[1] + gy) * ctx.nf[0] + gx;
+}
+
+// Accumulates the unit's points into the fp64 LDS tile (re, im planes).
+template <int NS>
+__device__ __forceinline__ void col_spread_into_tile(const SpreadCtx<float> &ctx,
+                                                     const StartTile<3, NS> &st,
+                                                     const float2 *__restrict__ c, double *re,
+                                                     double *im, float *stage_all) {
+  constexpr int S = col_side(NS);
+  constexpr int FOOT = S + NS - 1;  // footprint columns per dimension (<= 8)
+  constexpr int P = col_tile_xy<NS>();
+  constexpr int NSX = 8 / S;        // sub-columns per bin side
+  constexpr int NSUB = NSX * NSX;
+  constexpr int PLANE = P * P;
+  // Loop bounds and the window position are wave-uniform; readfirstlane
+  // keeps them in scalar registers so that the control flow stays scalar.
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = lane & 7, lj = lane >> 3;
+  const bool on = li < FOOT && lj < FOOT;
+  float *stage = stage_all + wave * (64 * COL_REC);
+  const int u0 = __builtin_amdgcn_readfirstlane(st.first);
+  const int u1 = u0 + __builtin_amdgcn_readfirstlane(st.npts);
+  const int *cs = ctx.colstart + int64_t(__builtin_amdgcn_readfirstlane(st.bin)) * NSUB;
+
+  for (int sub = wave; sub < NSUB; sub += COL_WAVES) {
+    const int p0 = __builtin_amdgcn_readfirstlane(max(cs[sub], u0));
+    const int p1 = __builtin_amdgcn_readfirstlane(min(cs[sub + 1], u1));
+    if (p0 >= p1) continue;
+    const int ox = (sub % NSX) * S, oy = (sub / NSX) * S;
+    // Tile cell of this lane's column at z = 0 (only used when `on`).
+    const int colcell = on ? (oy + lj) * P + ox + li : 0;
+    float ar[NS], ai[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) ar[k] = ai[k] = 0.f;
+    int zb = 0;  // plane held in register 0
+
+    for (int p = p0; p < p1; p += 64) {
+      const int n = min(64, p1 - p);
+      int lzv = 0;  // lane k: first z plane of point p + k
+      if (lane < n) {
+        const int q = p + lane;
+        const int j = ctx.idx[q];
+        float kx[NS], ky[NS], kz[NS];
+        const int s0 = start_and_kernel<float, NS>(coord(ctx, 0, q, j), ctx.nf[0], kx, ctx.kp);
+        const int s1 = start_and_kernel<float, NS>(coord(ctx, 1, q, j), ctx.nf[1], ky, ctx.kp);
+        const int s2 = start_and_kernel<float, NS>(coord(ctx, 2, q, j), ctx.nf[2], kz, ctx.kp);
+        const int lx = min(max(st.local_start(s0, 0, ctx.nf[0]) - ox, 0), S - 1);
+        const int ly = min(max(st.local_start(s1, 1, ctx.nf[1]) - oy, 0), S - 1);
+        const int lz = st.local_start(s2, 2, ctx.nf[2]);
+        float *r = stage + lane * COL_REC;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) r[t] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          r[lx + k] = kx[k];
+          r[8 + ly + k] = ky[k];
+          r[16 + k] = kz[k];
+        }
+        const float2 cv = c[j];
+        r[24] = cv.x;
+        r[25] = cv.y;
+        lzv = lz;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+      // Two-stage pipeline: the LDS reads of point k + 1 are issued before
+      // point k is accumulated (records past n are in bounds and unused).
+      ColPoint<NS> cur, nxt;
+      cur.load(stage, 0, li, lj);
+      for (int k = 0; k < n; k += 2) {
+        nxt.load(stage, min(k + 1, 63), li, lj);
+        col_accumulate<NS>(cur, __builtin_amdgcn_readlane(lzv, k), zb, ar, ai, re, im, colcell,
+                           on);
+        if (k + 1 >= n) break;
+        cur.load(stage, min(k + 2, 63), li, lj);
+        col_accumulate<NS>(nxt, __builtin_amdgcn_readlane(lzv, k + 1), zb, ar, ai, re, im,
+                           colcell, on);
+      }
+      // all lanes are done reading this batch's records before restaging
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    col_flush_all<NS>(zb, ar, ai, re, im, colcell, PLANE, on);
+  }
+}
+
+// Whole-bin column spreading in parity colors with a plain tile flush; bins
+// with more than dense_min points are left to spread_col_sub_kernel.
+template <int NS>
+__global__ void __launch_bounds__(64 * COL_WAVES)
+    spread_col_colored_kernel(SpreadCtx<float> ctx, const float2 *__restrict__ c,
+                              float2 *__restrict__ fw, int64_t nftot, int col0, int col1,
+                              int col2) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *re = reinterpret_cast<double *>(lds_raw);
+  StartTile<3, NS> st;
+  st.init_bin(ctx, colored_bin<3>(ctx, blockIdx.x, col0, col1, col2));
+  if (st.npts == 0 || st.npts > ctx.dense_min) return;
+  double *im = re + st.ntile;
+  float *stage = reinterpret_cast<float *>(im + st.ntile);
+  const int t = blockIdx.y;
+  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
+    re[n] = 0.0;
+    im[n] = 0.0;
+  }
+  __syncthreads();
+  col_spread_into_tile<NS>(ctx, st, c + int64_t(t) * ctx.M, re, im, stage);
+  __syncthreads();
+  float2 *g = fw + int64_t(t) * nftot;
+  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
+    const int64_t gi = col_grid_index<NS>(ctx, st, n);
+    float2 v = g[gi];
+    v.x += float(re[n]);
+    v.y += float(im[n]);
+    g[gi] = v;
+  }
+}
+
+// Column spreading of the dense bins' subproblems with an atomic tile flush.
+template <int NS>
+__global__ void __launch_bounds__(64 * COL_WAVES)
+    spread_col_sub_kernel(SpreadCtx<float> ctx, const float2 *__restrict__ c,
+                          float2 *__restrict__ fw, int64_t nftot) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *re = reinterpret_cast<double *>(lds_raw);
+  StartTile<3, NS> st;
+  if (!st.init_sub(ctx, blockIdx.x)) return;
+  double *im = re + st.ntile;
+  float *stage = reinterpret_cast<float *>(im + st.ntile);
+  const int t = blockIdx.y;
+  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
+    re[n] = 0.0;
+    im[n] = 0.0;
+  }
+  __syncthreads();
+  col_spread_into_tile<NS>(ctx, st, c + int64_t(t) * ctx.M, re, im, stage);
+  __syncthreads();
+  float2 *g = fw + int64_t(t) * nftot;
+  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
+    const float vr = float(re[n]), vi = float(im[n]);
+    if (vr == 0.f && vi == 0.f) continue;
+    hbm_add<float>(g + col_grid_index<NS>(ctx, st, n), vr, vi);
   }
 }
 
@@ -461,6 +735,31 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
     dim3 grid((ctx.M + SPREAD_THREADS - 1) / SPREAD_THREADS, blk);
     spread_gm_kernel<T, DIM, NS><<<grid, SPREAD_THREADS, 0, stream>>>(ctx, c, fw, nftot);
     launch_check("spread_gm_kernel");
+  } else if (ctx.colS) {
+    if constexpr (DIM == 3 && std::is_same<T, float>::value && NS <= COL_MAX_NS) {
+      if (ctx.colored) {
+        auto k = spread_col_colored_kernel<NS>;
+        allow_lds(k, ctx.lds_bytes);
+        for (int color = 0; color < 8; ++color) {
+          int col[3] = {color & 1, (color >> 1) & 1, (color >> 2) & 1};
+          int64_t nblk = 1;
+          for (int d = 0; d < 3; ++d) nblk *= (ctx.nbins[d] - col[d] + 1) / 2;
+          if (nblk == 0) continue;
+          k<<<dim3(unsigned(nblk), blk), 64 * COL_WAVES, ctx.lds_bytes, stream>>>(
+              ctx, c, fw, nftot, col[0], col[1], col[2]);
+          launch_check("spread_col_colored_kernel");
+        }
+      }
+      if (ctx.n_dense_sub > 0) {
+        auto k = spread_col_sub_kernel<NS>;
+        allow_lds(k, ctx.lds_bytes);
+        k<<<dim3(unsigned(ctx.n_dense_sub), blk), 64 * COL_WAVES, ctx.lds_bytes, stream>>>(
+            ctx, c, fw, nftot);
+        launch_check("spread_col_sub_kernel");
+      }
+    } else {
+      fail(FINUFFT_ERR_INVALID_ARGUMENT, "column spreading needs 3-D fp32 with ns <= 8");
+    }
   } else if (ctx.colored) {
     auto k = spread_bin_colored_kernel<T, DIM, NS>;
     allow_lds(k, ctx.lds_bytes);

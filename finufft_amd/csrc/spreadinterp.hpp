@@ -55,7 +55,29 @@ struct SpreadCtx {
   // (non-atomic) tile flush; requires per dimension an even number (>= 2) of
   // equal bins of at least 2 ceil(ns/2) + 1 cells.
   int colored = 0;
+
+  // Column-window spreading (3-D, fp32, ns <= 8): colS is the sub-column side
+  // (0 = off) and colstart the sorted start of every (bin, sub-column)
+  // (nbins_tot * nsub + 1 entries). Colored launches skip bins with more than
+  // dense_min points; those bins are covered by n_dense_sub subproblems
+  // (substart / sub2bin built over the dense bins only) with an atomic flush.
+  int colS = 0;
+  const int *colstart = nullptr;
+  int dense_min = 0;
+  int n_dense_sub = 0;
 };
+
+// Column-window spreading geometry. A wave owns the footprint of one
+// sub-column: 8 x 8 (x, y) grid columns, one per lane, for the points whose
+// first node lies in an S x S square, S the largest power of two with
+// S + ns - 1 <= 8.
+constexpr int COL_MAX_NS = 8;
+constexpr int COL_WAVES = 4;    // waves per workgroup
+constexpr int COL_REC = 28;     // floats staged per point
+inline constexpr int col_side(int ns) { return ns <= 5 ? 4 : (ns <= 7 ? 2 : 1); }
+inline constexpr size_t col_stage_bytes() {
+  return size_t(COL_WAVES) * 64 * COL_REC * sizeof(float);
+}
 
 // c (M x blk, contiguous per vector) -> fw (nf_tot x blk). fw must be zeroed
 // by the caller. Dispatches on ctx.dim / ctx.ns / ctx.method.

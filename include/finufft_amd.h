@@ -26,7 +26,9 @@ FINUFFT_EXPORT int cufinufftf_execute_adjoint(cufinufftf_plan d_plan, hipFloatCo
 /* Plan introspection; no reference counterpart (the reference exposes these
  * values only through opts.debug printing, src/cuda/makeplan.cu:116-118).
  * iout[16]: type, dim, ns, nf1, nf2, nf3, gpu_method, binsize x/y/z,
- *           batchsize, M, number of bins, 0, 0, 0.
+ *           batchsize, M, number of bins, sub-column side of the column-window
+ *           spreader (0 when unused), whole-bin parity-colored spreading (0/1),
+ *           number of dense-bin subproblems.
  * dout[4]:  beta, upsampfac, 0, 0. */
 FINUFFT_EXPORT int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout,
                                          double *dout);

@@ -38,34 +38,54 @@ def bin_keys(pts_c, nf, binsize):
     return key
 
 
-def start_bin_keys(pts_c, nf, binsize, ns, fold64=True):
-    """Bins of the first support node s = ceil(r - ns/2), wrapped into [0, nf),
-    with r folded in fp64 (fp32 plans) or in the points' own precision."""
-    nb = nbins(nf, binsize)
-    key = np.zeros(len(pts_c[0]), np.int64)
-    stride = 1
+def start_nodes(pts_c, nf, ns, fold64=True):
+    """First support node s = ceil(r - ns/2) per dimension, wrapped into
+    [0, nf), with r folded in fp64 (fp32 plans) or in the points' own
+    precision; non-finite r is pinned to 0."""
+    out = []
     for d, x in enumerate(pts_c):
         xx = x.astype(np.float64) if fold64 else x
         r = cfold.fold_rescale(xx, nf[d])
         r = np.where((r >= 0) & (r < nf[d]), r, 0)
         s = np.ceil(r - np.asarray(ns * 0.5, r.dtype)).astype(np.int64)
-        s = np.where(s < 0, s + nf[d], s)
+        out.append(np.where(s < 0, s + nf[d], s))
+    return out
+
+
+def start_bin_keys(pts_c, nf, binsize, ns, fold64=True, col_side=0):
+    """Bins of the first support node. With col_side (3-D), the key is refined
+    inside each bin to (sub-column, local z): sub-columns are col_side x
+    col_side first-node squares in (x, y), x fastest, and points of a
+    sub-column are ordered by the local z of their first node."""
+    nb = nbins(nf, binsize)
+    key = np.zeros(len(pts_c[0]), np.int64)
+    loc = []
+    stride = 1
+    for d, s in enumerate(start_nodes(pts_c, nf, ns, fold64)):
         b = np.clip(s // binsize[d], 0, nb[d] - 1)
+        loc.append(np.clip(s - b * binsize[d], 0, binsize[d] - 1))
         key += b * stride
         stride *= nb[d]
+    if col_side:
+        nsx, nsy = binsize[0] // col_side, binsize[1] // col_side
+        sub = np.minimum(loc[0] // col_side, nsx - 1) + nsx * np.minimum(loc[1] // col_side, nsy - 1)
+        key = (key * (nsx * nsy) + sub) * binsize[2] + loc[2]
     return key
 
 
-def bin_sort(pts_c, nf, binsize, start_ns=None, fold64=True):
+def bin_sort(pts_c, nf, binsize, start_ns=None, fold64=True, col_side=0):
     """(permutation, binstart) of the stable bin sort; start_ns selects bins of
-    the first support node for kernel width start_ns."""
+    the first support node for kernel width start_ns, col_side the sub-column
+    refinement of those bins (3-D)."""
     if start_ns is None:
         key = bin_keys(pts_c, nf, binsize)
+        bin_key = key
     else:
-        key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64)
+        key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64, col_side)
+        bin_key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64) if col_side else key
     perm = np.argsort(key, kind="stable").astype(np.int32)
     total = int(np.prod(nbins(nf, binsize)))
-    counts = np.bincount(key, minlength=total)
+    counts = np.bincount(bin_key, minlength=total)
     binstart = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     return perm, binstart
 

@@ -181,7 +181,8 @@ def test_sort_permutation_bit_exact(torch_gpu):
         # method 1: the reference's bins of the folded coordinate; the LDS
         # method: bins of the first support node
         ref_perm, ref_start = osort.bin_sort(pts_c, list(info["nf"]), list(info["binsize"]),
-                                             start_ns=None if method == 1 else info["ns"])
+                                             start_ns=None if method == 1 else info["ns"],
+                                             col_side=info["col_side"])
         assert np.array_equal(to_host(perm), ref_perm), f"method {method}"
         assert np.array_equal(to_host(binstart), ref_start), f"method {method}"
 
@@ -213,3 +214,97 @@ def test_spread_paths_agree(torch_gpu, shape, dtype):
 def test_interp_bins_vs_direct(torch_gpu, shape):
     err, tol = _run(torch_gpu, 2, shape, "complex64", 2, M=4000, seed=10)
     assert err < 25 * tol
+
+
+def _type1_case(torch, shape, tol, pts, c, **kw):
+    from finufft_amd import Plan
+
+    plan = Plan(1, shape, eps=tol, dtype="complex64", **kw)
+    plan.setpts(*[to_dev(torch, p) for p in pts])
+    f = to_host(plan.execute(to_dev(torch, c)))
+    ref = nudft.type1([p.astype(np.float64) for p in pts], c, plan._isign, shape)
+    return nudft.relerr(f, ref), plan.info()
+
+
+@pytest.mark.parametrize("tol,ns", [(1e-3, 4), (2e-4, 5), (1e-4, 6), (1e-6, 7), (1e-7, 8)])
+def test_column_spread_widths(torch_gpu, tol, ns):
+    """3-D fp32 column-window spreading at kernel widths 4..8 (sub-column
+    sides 4, 4, 2, 2, 1) against the direct sum, 25 tol (1e-6 absolute floor
+    where fp32 rounding dominates). The widths are those of the fp32 width
+    rule (finufft_amd_kernel_shape)."""
+    side = 4 if ns <= 5 else (2 if ns <= 7 else 1)
+    r = rng(21)
+    shape = (16, 24, 32)
+    M = 5000
+    pts = uniform_pts(r, 3, M, np.float32)
+    c = cnormal(r, (M,), np.complex64)
+    err, info = _type1_case(torch_gpu, shape, tol, pts, c)
+    print(tol, info, err)
+    assert info["ns"] == ns
+    assert info["col_side"] == side and info["colored"] == 1
+    assert err < max(25 * tol, 1e-6)
+
+
+def test_column_spread_not_colorable(torch_gpu):
+    """nf = 90 per dimension is not a multiple of the 8-node bins: every bin
+    runs as subproblems with an atomic flush."""
+    r = rng(22)
+    shape = (45, 45, 45)
+    M = 4000
+    pts = uniform_pts(r, 3, M, np.float32)
+    c = cnormal(r, (M,), np.complex64)
+    err, info = _type1_case(torch_gpu, shape, 1e-5, pts, c)
+    print(info, err)
+    assert info["col_side"] == 2 and info["colored"] == 0 and info["dense_subproblems"] > 0
+    assert err < 25e-5
+
+
+@pytest.mark.parametrize("maxsub", [1024, 64])
+def test_column_spread_dense_bins(torch_gpu, maxsub):
+    """A tight cluster puts more points in one bin than a whole-bin
+    workgroup takes (8 maxsub): those bins run as subproblems after the
+    colored launches."""
+    r = rng(23)
+    shape = (24, 24, 24)
+    M = 12000
+    pts = uniform_pts(r, 3, M, np.float32)
+    # centre 7 grid cells above 0 (nf = 48): first nodes in the middle of a bin
+    x0 = 7 * 2 * np.pi / 48
+    for d in range(3):
+        pts[d][:10000] = (x0 + 0.02 * r.standard_normal(10000)).astype(np.float32)
+    c = cnormal(r, (M,), np.complex64)
+    err, info = _type1_case(torch_gpu, shape, 1e-5, pts, c, gpu_maxsubprobsize=maxsub)
+    print(info, err)
+    assert info["col_side"] == 2 and info["colored"] == 1 and info["dense_subproblems"] > 0
+    assert err < 25e-5
+
+
+def test_column_spread_user_bins(torch_gpu):
+    """User bin sizes (8, 8, bz) keep the column spreader; the sort is the
+    sub-column order of the oracle."""
+    import ctypes
+
+    from finufft_amd import _lib
+    from oracle import sort as osort
+
+    torch = torch_gpu
+    r = rng(24)
+    shape = (16, 24, 32)
+    M = 6000
+    pts = uniform_pts(r, 3, M, np.float32)
+    c = cnormal(r, (M,), np.complex64)
+    err, info = _type1_case(torch, shape, 1e-5, pts, c, gpu_binsizex=8, gpu_binsizey=8,
+                            gpu_binsizez=4)
+    assert info["binsize"] == (8, 8, 4) and info["col_side"] == 2
+    assert err < 25e-5
+    from finufft_amd import Plan
+
+    plan = Plan(1, shape, eps=1e-5, dtype="complex64", gpu_binsizex=8, gpu_binsizey=8,
+                gpu_binsizez=4)
+    plan.setpts(*[to_dev(torch, p) for p in pts])
+    perm = torch.empty(M, dtype=torch.int32, device="cuda")
+    assert _lib.lib().finufft_amd_copy_sort(plan.handle, 0, ctypes.c_void_p(perm.data_ptr()),
+                                            None) == 0
+    ref_perm, _ = osort.bin_sort(pts[::-1], list(info["nf"]), [8, 8, 4], start_ns=info["ns"],
+                                 col_side=2)
+    assert np.array_equal(to_host(perm), ref_perm)

@@ -304,3 +304,51 @@ def test_start_bins_cover_support():
     s = np.where(s < 0, s + nf, s)
     off = key * b
     assert np.all((s - off >= 0) & (s - off < b))
+
+
+def test_subcolumn_sort_vs_bruteforce():
+    """Sub-column refinement of the start bins (3-D column spreading): bins in
+    x-fastest order, inside a bin the S x S first-node squares in x-fastest
+    order, inside a square increasing z of the first node, ties by index. The
+    brute force walks that nesting literally."""
+    r = np.random.default_rng(6)
+    M = 400
+    pts = [r.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(3)]
+    nf, bs, ns, S = [32, 24, 16], [8, 8, 4], 7, 2
+    perm, start = osort.bin_sort(pts, nf, bs, start_ns=ns, col_side=S)
+    s = osort.start_nodes(pts, nf, ns)
+    nb = [n // b for n, b in zip(nf, bs)]
+    order = []
+    for bz in range(nb[2]):
+        for by in range(nb[1]):
+            for bx in range(nb[0]):
+                for sy in range(bs[1] // S):
+                    for sx in range(bs[0] // S):
+                        for lz in range(bs[2]):
+                            for i in range(M):
+                                if (s[0][i] // bs[0] == bx and s[1][i] // bs[1] == by
+                                        and s[2][i] // bs[2] == bz
+                                        and (s[0][i] % bs[0]) // S == sx
+                                        and (s[1][i] % bs[1]) // S == sy
+                                        and s[2][i] % bs[2] == lz):
+                                    order.append(i)
+    assert np.array_equal(perm, np.array(order, np.int32))
+    # bin-level starts are those of the unrefined start bins
+    _, start0 = osort.bin_sort(pts, nf, bs, start_ns=ns)
+    assert np.array_equal(start, start0)
+
+
+def test_fp32_width_table():
+    """fp32 kernel widths used by the 3-D column spreader tests (the reference
+    rule evaluated with tol / 10 in fp32, src/cuda/makeplan.cu:43-119)."""
+    import ctypes
+
+    from finufft_amd import _lib
+
+    lib = _lib.lib()
+    for tol, want in [(1e-3, 4), (2e-4, 5), (1e-4, 6), (1e-6, 7), (1e-7, 8)]:
+        ns, beta = ctypes.c_int(), ctypes.c_double()
+        assert lib.finufft_amd_kernel_shape(ctypes.c_double(tol), 0, ctypes.c_double(2.0), 1,
+                                            ctypes.byref(ns), ctypes.byref(beta)) == 0
+        assert ns.value == want, tol
+        assert ns.value == kernel.kernel_shape(tol, 2.0, is_double=False)[0]
