@@ -495,6 +495,12 @@ __device__ __forceinline__ int64_t col_grid_index(const SpreadCtx<float> &ctx,
 }
 
 // Accumulates the unit's points into the fp64 LDS tile (re, im planes).
+//
+// A wave walks its sub-columns (wave, wave + COL_WAVES, ...) in batches of up
+// to 64 points. The sorted index and coordinates of the next batch are loaded
+// into registers while the current batch is processed, and the strength
+// gather of a batch is issued before its kernel evaluation, so that global
+// memory latency overlaps compute.
 template <int NS>
 __device__ __forceinline__ void col_spread_into_tile(const SpreadCtx<float> &ctx,
                                                      const StartTile<3, NS> &st,
@@ -517,68 +523,99 @@ __device__ __forceinline__ void col_spread_into_tile(const SpreadCtx<float> &ctx
   const int u1 = u0 + __builtin_amdgcn_readfirstlane(st.npts);
   const int *cs = ctx.colstart + int64_t(__builtin_amdgcn_readfirstlane(st.bin)) * NSUB;
 
-  for (int sub = wave; sub < NSUB; sub += COL_WAVES) {
-    const int p0 = __builtin_amdgcn_readfirstlane(max(cs[sub], u0));
-    const int p1 = __builtin_amdgcn_readfirstlane(min(cs[sub + 1], u1));
-    if (p0 >= p1) continue;
+  // First non-empty sub-column at or after sub; its point range in [a, b).
+  auto find_sub = [&](int sub, int &a, int &b) {
+    for (; sub < NSUB; sub += COL_WAVES) {
+      a = __builtin_amdgcn_readfirstlane(max(cs[sub], u0));
+      b = __builtin_amdgcn_readfirstlane(min(cs[sub + 1], u1));
+      if (a < b) break;
+    }
+    return sub;
+  };
+  // Sorted index and coordinates of one lane's point of the next batch,
+  // loaded while the current batch is processed.
+  int fj = 0;
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  auto fetch = [&](int q, int end) {
+    if (q < end) {
+      fj = ctx.idx[q];
+      fx = coord(ctx, 0, q, fj);
+      fy = coord(ctx, 1, q, fj);
+      fz = coord(ctx, 2, q, fj);
+    }
+  };
+
+  int p = 0, pe = 0;
+  int sub = find_sub(wave, p, pe);
+  if (sub < NSUB) fetch(p + lane, pe);
+  float ar[NS], ai[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) ar[k] = ai[k] = 0.f;
+  int zb = 0;  // plane held in register 0
+
+  while (sub < NSUB) {
+    const int n = min(64, pe - p);
+    const int j = fj;
+    const float x = fx, y = fy, z = fz;
+    // next batch: the rest of this sub-column or the next non-empty one
+    int np = p + 64, npe = pe, nsub = sub;
+    if (np >= pe) nsub = find_sub(sub + COL_WAVES, np, npe);
+    if (nsub < NSUB) fetch(np + lane, npe);
+
     const int ox = (sub % NSX) * S, oy = (sub / NSX) * S;
     // Tile cell of this lane's column at z = 0 (only used when `on`).
     const int colcell = on ? (oy + lj) * P + ox + li : 0;
-    float ar[NS], ai[NS];
+    int lzv = 0;  // lane k: first z plane of point p + k
+    if (lane < n) {
+      const float2 cv = c[j];  // issued before the kernel evaluation
+      float kx[NS], ky[NS], kz[NS];
+      const int s0 = start_and_kernel<float, NS>(x, ctx.nf[0], kx, ctx.kp);
+      const int s1 = start_and_kernel<float, NS>(y, ctx.nf[1], ky, ctx.kp);
+      const int s2 = start_and_kernel<float, NS>(z, ctx.nf[2], kz, ctx.kp);
+      const int lx = min(max(st.local_start(s0, 0, ctx.nf[0]) - ox, 0), S - 1);
+      const int ly = min(max(st.local_start(s1, 1, ctx.nf[1]) - oy, 0), S - 1);
+      lzv = st.local_start(s2, 2, ctx.nf[2]);
+      float *r = stage + lane * COL_REC;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) ar[k] = ai[k] = 0.f;
-    int zb = 0;  // plane held in register 0
-
-    for (int p = p0; p < p1; p += 64) {
-      const int n = min(64, p1 - p);
-      int lzv = 0;  // lane k: first z plane of point p + k
-      if (lane < n) {
-        const int q = p + lane;
-        const int j = ctx.idx[q];
-        float kx[NS], ky[NS], kz[NS];
-        const int s0 = start_and_kernel<float, NS>(coord(ctx, 0, q, j), ctx.nf[0], kx, ctx.kp);
-        const int s1 = start_and_kernel<float, NS>(coord(ctx, 1, q, j), ctx.nf[1], ky, ctx.kp);
-        const int s2 = start_and_kernel<float, NS>(coord(ctx, 2, q, j), ctx.nf[2], kz, ctx.kp);
-        const int lx = min(max(st.local_start(s0, 0, ctx.nf[0]) - ox, 0), S - 1);
-        const int ly = min(max(st.local_start(s1, 1, ctx.nf[1]) - oy, 0), S - 1);
-        const int lz = st.local_start(s2, 2, ctx.nf[2]);
-        float *r = stage + lane * COL_REC;
+      for (int t = 0; t < 16; ++t) r[t] = 0.f;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) r[t] = 0.f;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-          r[lx + k] = kx[k];
-          r[8 + ly + k] = ky[k];
-          r[16 + k] = kz[k];
-        }
-        const float2 cv = c[j];
-        r[24] = cv.x;
-        r[25] = cv.y;
-        lzv = lz;
+      for (int k = 0; k < NS; ++k) {
+        r[lx + k] = kx[k];
+        r[8 + ly + k] = ky[k];
+        r[16 + k] = kz[k];
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-
-      // Two-stage pipeline: the LDS reads of point k + 1 are issued before
-      // point k is accumulated (records past n are in bounds and unused).
-      ColPoint<NS> cur, nxt;
-      cur.load(stage, 0, li, lj);
-      for (int k = 0; k < n; k += 2) {
-        nxt.load(stage, min(k + 1, 63), li, lj);
-        col_accumulate<NS>(cur, __builtin_amdgcn_readlane(lzv, k), zb, ar, ai, re, im, colcell,
-                           on);
-        if (k + 1 >= n) break;
-        cur.load(stage, min(k + 2, 63), li, lj);
-        col_accumulate<NS>(nxt, __builtin_amdgcn_readlane(lzv, k + 1), zb, ar, ai, re, im,
-                           colcell, on);
-      }
-      // all lanes are done reading this batch's records before restaging
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      r[24] = cv.x;
+      r[25] = cv.y;
     }
-    col_flush_all<NS>(zb, ar, ai, re, im, colcell, PLANE, on);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+    // Two-stage pipeline: the LDS reads of point k + 1 are issued before
+    // point k is accumulated (records past n are in bounds and unused).
+    ColPoint<NS> cur, nxt;
+    cur.load(stage, 0, li, lj);
+    for (int k = 0; k < n; k += 2) {
+      nxt.load(stage, min(k + 1, 63), li, lj);
+      col_accumulate<NS>(cur, __builtin_amdgcn_readlane(lzv, k), zb, ar, ai, re, im, colcell,
+                         on);
+      if (k + 1 >= n) break;
+      cur.load(stage, min(k + 2, 63), li, lj);
+      col_accumulate<NS>(nxt, __builtin_amdgcn_readlane(lzv, k + 1), zb, ar, ai, re, im,
+                         colcell, on);
+    }
+    // all lanes are done reading this batch's records before restaging
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+    if (nsub != sub) {  // sub-column done: hand the window to the tile
+      col_flush_all<NS>(zb, ar, ai, re, im, colcell, PLANE, on);
+      zb = 0;
+    }
+    sub = nsub;
+    p = np;
+    pe = npe;
   }
 }
 
