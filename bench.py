@@ -89,6 +89,105 @@ def cpu_baseline(N, tol, M_full, M_sample, threads):
     }
 
 
+def batched_comparison(nprob, ntr, tol, reps, log_every=500):
+    """The uneven small-problem workload: nprob independent 2-D type-2
+    problems, side N log-uniform in [32, 512], M log-uniform in [1e3, 1e5]
+    points uniform on [-pi, pi)^2, ntr complex normal coefficient vectors each,
+    fp32 at tol. Times a loop of per-plan executes against the fused batch
+    (one amplification launch, one FFT per grid shape and one persistent
+    interpolation launch per round, replayed as a HIP graph). Returns the
+    speedup, both throughputs (NU point evaluations per second, M ntr per
+    problem) and the errors at sampled points of sampled problems."""
+    import numpy as np
+    import torch
+
+    from finufft_amd import Plan
+    from finufft_amd import accuracy as acc
+    from finufft_amd.batch import BatchPlan
+
+    dev = torch.device("cuda", 0)
+    r = np.random.default_rng(3)
+    Ns = np.rint(np.exp(r.uniform(np.log(32), np.log(512), nprob))).astype(int)
+    Ms = np.rint(np.exp(r.uniform(np.log(1e3), np.log(1e5), nprob))).astype(int)
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    plans, pts, fks, outs = [], [], [], []
+    t0 = time.perf_counter()
+    for i, (n, m) in enumerate(zip(Ns, Ms)):
+        x = [torch.rand(int(m), generator=g, device=dev) * (2 * math.pi) - math.pi
+             for _ in range(2)]
+        fk = torch.complex(torch.randn((ntr, n, n), generator=g, device=dev),
+                           torch.randn((ntr, n, n), generator=g, device=dev))
+        p = Plan(2, (int(n), int(n)), n_trans=ntr, eps=tol, dtype="complex64", isign=-1)
+        p.setpts(*x)
+        plans.append(p)
+        pts.append(x)
+        fks.append(fk)
+        outs.append(torch.empty((ntr, int(m)), dtype=torch.complex64, device=dev))
+        if log_every and (i + 1) % log_every == 0:
+            log(f"batched workload: {i + 1}/{nprob} problems planned")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+
+    def plain():
+        for p, fk, o in zip(plans, fks, outs):
+            p.execute(fk, out=o)
+
+    plain()  # FFT plans, page-in
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    samp = r.choice(nprob, min(8, nprob), replace=False)
+    pidx = {int(i): torch.randint(0, int(Ms[i]), (32,), device=dev) for i in samp}
+
+    def errors():
+        e = 0.0
+        for i in samp:
+            ref = acc.type2_at(pts[i], fks[i][0], -1, pidx[int(i)])
+            e = max(e, acc.relerr(outs[i][0][pidx[int(i)]], ref))
+        return e
+
+    e_plain = errors()
+    log(f"batched workload: setup {t_setup:.1f} s, per-plan loop {t_plain * 1e3:.1f} ms")
+
+    t0 = time.perf_counter()
+    b = BatchPlan(plans)
+    for o in outs:
+        o.zero_()
+    b.execute(fks, outs)  # records the graph
+    torch.cuda.synchronize()
+    t_bsetup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.execute(fks, outs)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    e_batch = errors()
+    info = b.info()
+    evals = float(np.sum(Ms)) * ntr
+    log(f"batched workload: fused {t_batch * 1e3:.1f} ms ({info})")
+    del b
+    return {
+        "workload": (f"{nprob} 2D type-2 problems, N log-uniform [32,512], M log-uniform "
+                     f"[1e3,1e5], ntransf {ntr}, fp32 tol {tol:g}"),
+        "speedup": t_plain / t_batch,
+        "plain_ms": round(t_plain * 1e3, 3),
+        "batched_ms": round(t_batch * 1e3, 3),
+        "plain_nu_evals_per_s": evals / t_plain,
+        "batched_nu_evals_per_s": evals / t_batch,
+        "error_plain": e_plain,
+        "error_batched": e_batch,
+        "setup_s": round(t_setup, 2),
+        "batch_setup_s": round(t_bsetup, 2),
+        "vectors_per_round": info["vectors_per_round"],
+        "grid_shapes": info["grid_shapes"],
+        "graph": info["graph"],
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +205,10 @@ def main(argv=None):
                     help="gpu_binsize x,y,z of the type-1 plan (default: the plan's choice)")
     ap.add_argument("--binsize2", type=str, default="",
                     help="gpu_binsize x,y,z of the type-2 plan (default: the plan's choice)")
+    ap.add_argument("--batched", type=int, default=2000,
+                    help="problems of the uneven small-problem comparison (0: skip)")
+    ap.add_argument("--batched-ntr", type=int, default=32)
+    ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a tensor of this many MiB (a plain copy kernel of "
                          "known size, the reference point for HBM counter readings)")
@@ -221,6 +324,13 @@ def main(argv=None):
         log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del src, dst
 
+    batched = None
+    if args.batched > 0:
+        del p1, p2, pts, c, f, fk_out, c_out
+        torch.cuda.empty_cache()
+        batched = batched_comparison(args.batched, args.batched_ntr, 1e-5, args.batched_reps)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -257,6 +367,7 @@ def main(argv=None):
                       "type2": {k: round(v, 4) for k, v in s2.items()}},
         "setpts_ms": {"type1": round(t_set1 * 1e3, 2), "type2": round(t_set2 * 1e3, 2)},
         "error": {"type1": e1, "type2": e2, "samples": S},
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_double, c_float, c_int, c_int64, c_void_p
+from ctypes import POINTER, c_double, c_float, c_int, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
 try:  # noqa: SIM105 - import order matters, see the module docstring
@@ -68,9 +68,29 @@ def _declare(lib):
             f = getattr(lib, f"{prefix}_{name}")
             f.argtypes = [vp, vp, vp]
             f.restype = c_int
+        g = getattr(lib, f"{prefix}_interp_grad")
+        g.argtypes = [vp, vp, vp, vp]
+        g.restype = c_int
         d = getattr(lib, f"{prefix}_destroy")
         d.argtypes = [vp]
         d.restype = c_int
+    lib.finufft_amd_batch_makeplan.argtypes = [c_int, c_int, POINTER(c_void_p), c_size_t,
+                                               POINTER(c_void_p)]
+    lib.finufft_amd_batch_makeplan.restype = c_int
+    lib.finufft_amd_batch_execute.argtypes = [vp, POINTER(c_void_p), POINTER(c_void_p)]
+    lib.finufft_amd_batch_execute.restype = c_int
+    lib.finufft_amd_batch_destroy.argtypes = [vp]
+    lib.finufft_amd_batch_destroy.restype = c_int
+    lib.finufft_amd_batch_info.argtypes = [vp, i64p]
+    lib.finufft_amd_batch_info.restype = c_int
+    lib.finufft_amd_batch_set_graph.argtypes = [vp, c_int]
+    lib.finufft_amd_batch_set_graph.restype = c_int
+    lib.finufft_amd_mlp_forward.argtypes = [vp, c_int, vp, vp, vp]
+    lib.finufft_amd_mlp_forward.restype = c_int
+    lib.finufft_amd_mlp_grad.argtypes = [vp, c_int, vp, vp, vp, vp, vp, vp]
+    lib.finufft_amd_mlp_grad.restype = c_int
+    lib.finufft_amd_mlp_adam.argtypes = [vp, vp, vp, vp, c_int, c_float, vp]
+    lib.finufft_amd_mlp_adam.restype = c_int
     lib.finufft_amd_plan_info.argtypes = [vp, c_int, i64p, POINTER(c_double)]
     lib.finufft_amd_plan_info.restype = c_int
     lib.finufft_amd_copy_sort.argtypes = [vp, c_int, vp, vp]

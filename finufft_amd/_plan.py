@@ -257,6 +257,36 @@ class Plan:
         """The adjoint transform on the same points (types 1 and 2)."""
         return self._run(data, out, adjoint=True)
 
+    def interp_grad(self, fk, value=True):
+        """The type-2 direction of the plan (type 2: the transform, type 1: the
+        adjoint) applied to the modes fk, and the derivatives of the result
+        with respect to the point coordinates.
+
+        Returns (c, dc): c has the shape of the values (None when value is
+        False); dc[k] is the derivative with respect to the k-th coordinate
+        array passed to setpts (x, y, z), each with the shape of c.
+        """
+        if self._type == 3:
+            raise ValueError("interp_grad is provided for types 1 and 2")
+        fk = self._check(fk, "fk", False)
+        modes = tuple(self._n_modes)
+        n = self._n_trans
+        if fk.shape == modes and n == 1:
+            batch = ()
+        elif fk.shape == (n, *modes) or (n == 1 and fk.shape == (1, *modes)):
+            batch = (n,)
+        else:
+            raise ValueError(f"Invalid fk shape {tuple(fk.shape)}")
+        c = torch.empty(batch + (self._nj,), dtype=fk.dtype, device=fk.device) if value else None
+        dc = torch.empty((self._dim,) + batch + (self._nj,), dtype=fk.dtype, device=fk.device)
+        fn = f"{self._pref}_interp_grad"
+        ier = getattr(_lib.lib(), fn)(self._plan, c.data_ptr() if value else None,
+                                      dc.data_ptr(), fk.data_ptr())
+        if ier != 0:
+            raise FinufftError(fn, ier)
+        # the library's dimension 0 is x, the last array axis
+        return c, dc.flip(0)
+
     def __del__(self):
         if getattr(self, "_plan", None):
             try:

@@ -1,0 +1,78 @@
+// Fused execution of many independent NUFFT problems.
+//
+// The reference runs one plan per problem, with one launch per stage and
+// vector batch (src/cuda/execute.cu:105-204); for thousands of small, uneven
+// problems the GPU then spends its time between launches. A BatchPlan takes
+// a set of type-2 plans whose points are already set and runs all of them
+// per round of vectors as
+//   * one amplification launch over every problem (fk -> fine grid),
+//   * one batched rocFFT call per distinct fine-grid shape (problems of equal
+//     shape are contiguous in a shared arena),
+//   * one persistent interpolation launch over every problem's subproblems,
+// all recorded once into a hipGraph and replayed on every execute. The
+// per-problem input/output pointers live in a small device table that is
+// refreshed before each replay, so the graph never changes.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "fft.hpp"
+#include "plan.hpp"
+#include "plan_kernels.hpp"
+
+namespace fa {
+
+template <typename T>
+class BatchPlan {
+ public:
+  using C = cplx_t<T>;
+
+  // plans: type-2 plans with points set, of equal dimension, ntransf, iflag
+  // and kernel width. arena_bytes bounds the shared fine-grid arena (the
+  // vectors of a problem are processed in rounds of V, V dividing ntransf).
+  BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStream_t stream);
+  ~BatchPlan();
+  BatchPlan(const BatchPlan &) = delete;
+  BatchPlan &operator=(const BatchPlan &) = delete;
+
+  // c[p]: values of problem p (ntransf x M_p), fk[p]: its modes
+  // (ntransf x N_p); device pointers, host arrays of length nprob.
+  void execute(C *const *c, C *const *fk);
+
+  int nprob() const { return int(plans_.size()); }
+  int vectors_per_round() const { return V_; }
+  int groups() const { return int(groups_.size()); }
+  bool graph() const { return gexec_ != nullptr; }
+  void set_use_graph(bool on) { use_graph_ = on; }
+
+ private:
+  struct Group {
+    int nf[3];
+    std::vector<int> probs;
+    int64_t off = 0;  // cells
+    FftPlan fft;
+  };
+  void enqueue();
+
+  std::vector<Plan<T> *> plans_;
+  hipStream_t stream_ = nullptr;
+  int dim_ = 0, ntr_ = 1, iflag_ = 1, ns_ = 2, V_ = 1;
+  size_t lds_bytes_ = 0;
+  dbuf<C> arena_;
+  std::vector<std::unique_ptr<Group>> groups_;
+  dbuf<AmpItem<T>> amp_items_;
+  dbuf<InterpItem<T>> interp_items_;
+  int64_t amp_blocks_ = 0, interp_units_ = 0;
+  dbuf<C *> fw_base_;
+  dbuf<const C *> fk_base_;
+  dbuf<C *> c_base_;
+  std::vector<const C *> fk_host_;
+  std::vector<C *> c_host_;
+  bool use_graph_ = true;
+  bool graph_failed_ = false;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+};
+
+}  // namespace fa

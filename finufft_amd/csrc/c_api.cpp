@@ -14,7 +14,9 @@
 #include <limits>
 #include <memory>
 
+#include "batch.hpp"
 #include "kernel_math.hpp"
+#include "mlp.hpp"
 #include "plan.hpp"
 
 using fa::Plan;
@@ -102,6 +104,28 @@ fa::cplx_t<T> *nc(const fa::cplx_t<T> *p) {
 
 }  // namespace
 
+namespace {
+// A batch handle remembers its precision.
+struct BatchHandle {
+  int is_double;
+  std::unique_ptr<fa::BatchPlan<float>> f;
+  std::unique_ptr<fa::BatchPlan<double>> d;
+};
+BatchHandle *as_batch(void *b) {
+  if (!b) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null batch");
+  return static_cast<BatchHandle *>(b);
+}
+template <typename T>
+std::unique_ptr<fa::BatchPlan<T>> make_batch(int n, void *const *plans, size_t arena) {
+  if (n < 1 || plans == nullptr) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: no plans");
+  std::vector<Plan<T> *> v(n);
+  for (int i = 0; i < n; ++i) v[i] = as_plan<T>(plans[i]);
+  const size_t budget = arena ? arena : (size_t(16) << 30);
+  const hipStream_t stream = v[0]->stream();
+  return std::make_unique<fa::BatchPlan<T>>(std::move(v), budget, stream);
+}
+}  // namespace
+
 extern "C" {
 
 void cufinufft_default_opts(cufinufft_opts *opts) {
@@ -164,6 +188,100 @@ int cufinufft_execute_adjoint(cufinufft_plan p, hipDoubleComplex *c, hipDoubleCo
 }
 int cufinufftf_execute_adjoint(cufinufftf_plan p, hipFloatComplex *c, hipFloatComplex *fk) {
   return safe_call([&] { as_plan<float>(p)->execute_adjoint(c, fk); });
+}
+
+int cufinufft_interp_grad(cufinufft_plan p, hipDoubleComplex *c, hipDoubleComplex *dc,
+                          hipDoubleComplex *fk) {
+  return safe_call([&] { as_plan<double>(p)->interp_grad(c, dc, fk); });
+}
+int cufinufftf_interp_grad(cufinufftf_plan p, hipFloatComplex *c, hipFloatComplex *dc,
+                           hipFloatComplex *fk) {
+  return safe_call([&] { as_plan<float>(p)->interp_grad(c, dc, fk); });
+}
+
+
+int finufft_amd_batch_makeplan(int is_double, int nplans, void *const *plans, size_t arena_bytes,
+                               finufft_amd_batch *out) {
+  return safe_call([&] {
+    if (!out) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: null output handle");
+    *out = nullptr;
+    auto h = std::make_unique<BatchHandle>();
+    h->is_double = is_double ? 1 : 0;
+    if (is_double)
+      h->d = make_batch<double>(nplans, plans, arena_bytes);
+    else
+      h->f = make_batch<float>(nplans, plans, arena_bytes);
+    *out = h.release();
+  });
+}
+
+int finufft_amd_batch_execute(finufft_amd_batch batch, void *const *c, void *const *fk) {
+  return safe_call([&] {
+    BatchHandle *h = as_batch(batch);
+    if (!c || !fk) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: null pointer array");
+    if (h->is_double)
+      h->d->execute(reinterpret_cast<double2 *const *>(c), reinterpret_cast<double2 *const *>(fk));
+    else
+      h->f->execute(reinterpret_cast<float2 *const *>(c), reinterpret_cast<float2 *const *>(fk));
+  });
+}
+
+int finufft_amd_batch_destroy(finufft_amd_batch batch) {
+  return safe_call([&] { delete as_batch(batch); });
+}
+
+int finufft_amd_batch_info(finufft_amd_batch batch, int64_t *iout) {
+  return safe_call([&] {
+    BatchHandle *h = as_batch(batch);
+    if (!iout) return;
+    auto fill = [&](const auto &b) {
+      iout[0] = b.nprob();
+      iout[1] = b.vectors_per_round();
+      iout[2] = b.groups();
+      iout[3] = b.graph() ? 1 : 0;
+    };
+    if (h->is_double)
+      fill(*h->d);
+    else
+      fill(*h->f);
+  });
+}
+
+int finufft_amd_batch_set_graph(finufft_amd_batch batch, int use_graph) {
+  return safe_call([&] {
+    BatchHandle *h = as_batch(batch);
+    if (h->is_double)
+      h->d->set_use_graph(use_graph != 0);
+    else
+      h->f->set_use_graph(use_graph != 0);
+  });
+}
+
+int finufft_amd_mlp_forward(const float *params, int n, const float *x, float *y,
+                            void *stream) {
+  return safe_call([&] {
+    if (n < 0 || (n > 0 && (!params || !x || !y)))
+      fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "mlp_forward: invalid arguments");
+    fa::mlp_forward(params, n, x, y, static_cast<hipStream_t>(stream));
+  });
+}
+
+int finufft_amd_mlp_grad(const float *params, int n, const float *x, const float *target,
+                         const float *mask, float *grad, float *loss, void *stream) {
+  return safe_call([&] {
+    if (n < 0 || (n > 0 && (!params || !x || !target || !mask || !grad || !loss)))
+      fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "mlp_grad: invalid arguments");
+    fa::mlp_grad(params, n, x, target, mask, grad, loss, static_cast<hipStream_t>(stream));
+  });
+}
+
+int finufft_amd_mlp_adam(float *params, const float *grad, float *m1, float *m2, int step,
+                         float lr, void *stream) {
+  return safe_call([&] {
+    if (!params || !grad || !m1 || !m2 || step < 1)
+      fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "mlp_adam: invalid arguments");
+    fa::mlp_adam(params, grad, m1, m2, step, lr, static_cast<hipStream_t>(stream));
+  });
 }
 
 int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout, double *dout) {

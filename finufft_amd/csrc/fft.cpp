@@ -63,6 +63,11 @@ rocfft_plan_t *FftPlan::plan_for(int sign) {
   return p;
 }
 
+void FftPlan::prepare(int sign) {
+  if (!ready()) fail(FINUFFT_ERR_CUDA_FAILURE, "[finufft_amd] FFT used before setup");
+  plan_for(sign);
+}
+
 void FftPlan::exec(void *data, int sign, hipStream_t stream) {
   if (!ready()) fail(FINUFFT_ERR_CUDA_FAILURE, "[finufft_amd] FFT used before setup");
   rocfft_plan_t *p = plan_for(sign);

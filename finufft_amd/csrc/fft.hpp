@@ -27,6 +27,9 @@ class FftPlan {
   // In place on batch contiguous vectors; the plan for each sign is created
   // on first use.
   void exec(void *data, int sign, hipStream_t stream);
+  // Creates the plan of one sign and its work buffer now (before a stream
+  // capture, where plan creation must not happen).
+  void prepare(int sign);
   bool ready() const { return dim_ > 0; }
 
  private:

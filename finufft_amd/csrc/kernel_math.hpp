@@ -179,4 +179,67 @@ __device__ __forceinline__ int start_and_kernel(T x, int nf, T *ker, const KerPa
   return s;
 }
 
+// Kernel values and their derivatives with respect to the offset xl: the
+// Horner polynomial p(z), z = 2 xl + ns - 1, gives d/dxl = 2 p'(z); the direct
+// form phi(a) = exp(beta (sqrt(1 - c a^2) - 1)), a = xl + i, gives
+// phi'(a) = -phi(a) beta c a / sqrt(1 - c a^2) (0 outside the support).
+template <typename T, int NS>
+__device__ __forceinline__ void eval_kernel_d(T *ker, T *dker, T xl, T z,
+                                              const KerParams<T> &kp) {
+  if (kp.horner) {
+    constexpr int NC = horner_nc_sigma2(NS);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      ker[i] = kp.coef[(NC - 1) * NS + i];
+      dker[i] = T(0);
+    }
+#pragma unroll
+    for (int k = NC - 2; k >= 0; --k) {
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        dker[i] = std::fma(dker[i], z, ker[i]);
+        ker[i] = std::fma(ker[i], z, kp.coef[k * NS + i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) dker[i] *= T(2);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const T a = xl + T(i);
+      const T zsq = kp.es_c * a * a;
+      if (zsq < T(1)) {
+        const T root = std::sqrt(T(1) - zsq);
+        ker[i] = std::exp(kp.beta * (root - T(1)));
+        dker[i] = -ker[i] * kp.beta * kp.es_c * a / root;
+      } else {
+        ker[i] = T(0);
+        dker[i] = T(0);
+      }
+    }
+  }
+}
+
+// start_and_kernel with the derivatives d ker / d xl (xl = s - x_rescaled).
+template <typename T, int NS>
+__device__ __forceinline__ int start_and_kernel_d(T x, int nf, T *ker, T *dker,
+                                                  const KerParams<T> &kp) {
+  if constexpr (sizeof(T) == 4) {
+    if (kp.fold64) {
+      double r = fold_rescale(double(x), nf);
+      r = (r >= 0.0 && r < double(nf)) ? r : 0.0;
+      const int s = int(std::ceil(r - NS * 0.5));
+      const double xl = double(s) - r;
+      eval_kernel_d<T, NS>(ker, dker, T(xl), T(std::fma(2.0, xl, double(NS - 1))), kp);
+      return s;
+    }
+  }
+  T r = fold_rescale(x, nf);
+  r = (r >= T(0) && r < T(nf)) ? r : T(0);
+  const int s = int(std::ceil(r - T(NS * 0.5)));
+  const T xl = T(s) - r;
+  eval_kernel_d<T, NS>(ker, dker, xl, std::fma(T(2), xl, T(NS - 1)), kp);
+  return s;
+}
+
 }  // namespace fa

@@ -34,6 +34,40 @@ void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
   }
 }
 
+template <typename T>
+void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns,
+                  const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
+                  int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                  hipStream_t stream) {
+  switch (dim) {
+    case 1:
+      return interp_batch_dim<T, 1>(items, nprob, ns, fw_base, c_base, c_vec_offset, total_units,
+                                    lds_bytes, stream);
+    case 2:
+      return interp_batch_dim<T, 2>(items, nprob, ns, fw_base, c_base, c_vec_offset, total_units,
+                                    lds_bytes, stream);
+    default:
+      return interp_batch_dim<T, 3>(items, nprob, ns, fw_base, c_base, c_vec_offset, total_units,
+                                    lds_bytes, stream);
+  }
+}
+template void interp_batch<float>(const InterpItem<float> *, int, int, int,
+                                  const cplx_t<float> *const *, cplx_t<float> *const *, int64_t,
+                                  int64_t, size_t, hipStream_t);
+template void interp_batch<double>(const InterpItem<double> *, int, int, int,
+                                   const cplx_t<double> *const *, cplx_t<double> *const *,
+                                   int64_t, int64_t, size_t, hipStream_t);
+
+template <typename T>
+void interp_grad(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
+                 int blk, hipStream_t stream) {
+  switch (ctx.dim) {
+    case 1: return interp_grad_dim<T, 1>(ctx, c, dc, fw, blk, stream);
+    case 2: return interp_grad_dim<T, 2>(ctx, c, dc, fw, blk, stream);
+    default: return interp_grad_dim<T, 3>(ctx, c, dc, fw, blk, stream);
+  }
+}
+
 namespace {
 
 int device_lds_limit(int dev) {
@@ -148,6 +182,25 @@ bool choose_col_bins(const int *nf, int ns, size_t budget, size_t limit, int *ou
       *colored = colr;
       return true;
     }
+  }
+  return false;
+}
+
+// 3-D interpolation bins: x-elongated shapes (wide rows spread a wave's
+// gathers over more LDS banks), the first in order of preference whose tile
+// fits `budget` (two workgroups per CU) and that still gives min_units bins.
+bool choose_interp_bins_3d(const int *nf, int ns, size_t cell_bytes, size_t budget,
+                           int64_t min_units, int *out) {
+  static const int pref[][3] = {{32, 8, 8}, {32, 8, 4}, {16, 8, 8}, {16, 8, 4},
+                                {16, 4, 4}, {8, 8, 8},  {8, 4, 4},  {4, 4, 4}};
+  for (const auto &b : pref) {
+    if (b[0] > nf[0] || b[1] > nf[1] || b[2] > nf[2]) continue;
+    if (size_t(tile_cells(3, b, ns)) * cell_bytes > budget) continue;
+    int64_t units = 1;
+    for (int d = 0; d < 3; ++d) units *= (nf[d] + b[d] - 1) / b[d];
+    if (units < min_units) continue;
+    for (int d = 0; d < 3; ++d) out[d] = b[d];
+    return true;
   }
   return false;
 }
@@ -282,6 +335,7 @@ void Plan<T>::setup_bins() {
     lds_bytes_ = 0;
   };
   geom_.colS = 0;
+  geom_.bank = 0;
   // Column-window spreading: 3-D fp32 spreading plans with ns <= 8. User bin
   // sizes select it when x and y are 8 (the footprint of one wave).
   const bool col_ok = opts_.gpu_method != 1 && type_ != 2 && dim_ == 3 &&
@@ -321,7 +375,11 @@ void Plan<T>::setup_bins() {
       ok = size_t(tile_cells(dim_, bs, ns_)) * cell <= size_t(lds_limit);
     } else {
       const int64_t min_units = 1024;
-      ok = spreads && choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), true, min_units, bs);
+      ok = !spreads && dim_ == 3 &&
+           choose_interp_bins_3d(nf_, ns_, cell, std::min<size_t>(size_t(lds_limit), 64 * 1024),
+                                 min_units, bs);
+      if (!ok)
+        ok = spreads && choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), true, min_units, bs);
       if (!ok) ok = choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), false, min_units, bs);
     }
     if (!ok) {
@@ -335,6 +393,8 @@ void Plan<T>::setup_bins() {
       geom_.start_bins = 1;
       colored_ = spreads && colorable(dim_, nf_, bs, ns_);
       lds_bytes_ = size_t(tile_cells(dim_, bs, ns_)) * cell;
+      // interpolation plans order each bin by LDS bank class
+      geom_.bank = spreads ? 0 : 1;
     }
   }
   for (int d = 0; d < 3; ++d)
@@ -403,15 +463,18 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
     }
   }
   if (opts_.gpu_method != 1) {
-    sub_capacity_ = subproblem_capacity(M_, nb, opts_.gpu_maxsubprobsize);
     substart_.reserve_at_least(size_t(nb) + 1);
-    sub2bin_.reserve_at_least(std::max(sub_capacity_, 1));
     // Column spreading runs whole bins in parity colors and only the bins
     // denser than dense_min_ (all bins when the shape is not colorable) as
     // subproblems; their count is read back so that the launch is exact.
     dense_min_ = geom_.colS && colored_ ? 8 * opts_.gpu_maxsubprobsize : 0;
-    build_subproblems(binstart_.get(), nb, opts_.gpu_maxsubprobsize, dense_min_,
-                      substart_.get(), sub2bin_.get(), scratch_, stream_);
+    // Interpolation plans cut bins into subproblems of sub_size_ points (one
+    // per bin for typical densities, several for dense bins).
+    sub_size_ = type_ == 2 ? 4 * opts_.gpu_maxsubprobsize : opts_.gpu_maxsubprobsize;
+    sub_capacity_ = subproblem_capacity(M_, nb, sub_size_);
+    sub2bin_.reserve_at_least(std::max(sub_capacity_, 1));
+    build_subproblems(binstart_.get(), nb, sub_size_, dense_min_, substart_.get(),
+                      sub2bin_.get(), scratch_, stream_);
     n_dense_sub_ = 0;
     if (geom_.colS) {
       hip_check(hipMemcpyAsync(&n_dense_sub_, substart_.get() + nb, sizeof(int),
@@ -533,7 +596,8 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.kp.coef = coef_.get();
   c.idx = idx_.get();
   c.binstart = binstart_.get();
-  c.maxsub = opts_.gpu_maxsubprobsize;
+  c.maxsub = sub_size_;
+  c.interp_sub = type_ == 2 && opts_.gpu_method != 1 ? 1 : 0;
   c.substart = substart_.get();
   c.sub2bin = sub2bin_.get();
   c.sub_capacity = sub_capacity_;
@@ -626,6 +690,44 @@ void Plan<T>::execute_adjoint(C *c, C *fk) {
   if (type_ == 1) return exec_type2(c, fk, -iflag_, ntransf_);
   if (type_ == 2) return exec_type1(c, fk, -iflag_, ntransf_);
   fail(FINUFFT_ERR_TYPE_NOTVALID, "adjoint is provided for types 1 and 2");
+}
+
+template <typename T>
+void Plan<T>::interp_grad(C *d_c, C *d_dc, C *d_fk) {
+  // The type-2 direction of the plan (the transform itself for type 2, the
+  // adjoint for type 1), with the coordinate derivatives of the result.
+  device_guard dg(opts_.gpu_device_id);
+  if (type_ == 3) fail(FINUFFT_ERR_TYPE_NOTVALID, "interp_grad is provided for types 1 and 2");
+  if (spread_only_) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad needs the full transform");
+  if (d_dc == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad: dc is null");
+  const int fft_sign = type_ == 2 ? iflag_ : -iflag_;
+  const SpreadCtx<T> ctx = spread_ctx();
+  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
+  const int64_t mtot = int64_t(M_) * ntransf_;
+  for (int i = 0; i * batchsize_ < ntransf_; ++i) {
+    const int blk = std::min(ntransf_ - i * batchsize_, batchsize_);
+    C *c = d_c ? d_c + int64_t(i) * batchsize_ * M_ : nullptr;
+    C *fk = d_fk + int64_t(i) * batchsize_ * nmodes_;
+    C *fw = fw_.get();
+    deconvolve<T>(dim_, opts_.modeord, false, ms_, nf_, ker, fw, fk, blk, stream_);
+    fft_.exec(fw, fft_sign, stream_);
+    // dc is (dim, ntransf, M); this batch writes vectors [i b, i b + blk) of
+    // every dimension, so the kernel's per-dimension stride must be mtot.
+    if (blk == ntransf_) {
+      ::fa::interp_grad<T>(ctx, c, d_dc, fw, blk, stream_);
+    } else {
+      for (int t = 0; t < blk; ++t) {
+        grad_scratch_.reserve_at_least(size_t(dim_) * M_);
+        ::fa::interp_grad<T>(ctx, c ? c + int64_t(t) * M_ : nullptr, grad_scratch_.get(),
+                             fw + int64_t(t) * nftot_, 1, stream_);
+        for (int d = 0; d < dim_; ++d)
+          hip_check(hipMemcpyAsync(d_dc + d * mtot + (int64_t(i) * batchsize_ + t) * M_,
+                                   grad_scratch_.get() + int64_t(d) * M_, size_t(M_) * sizeof(C),
+                                   hipMemcpyDeviceToDevice, stream_),
+                    "hipMemcpyAsync(grad)");
+      }
+    }
+  }
 }
 
 template class Plan<float>;

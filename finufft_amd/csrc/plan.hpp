@@ -5,8 +5,10 @@
 // (include/cufinufft/cufinufft_plan_t.hpp:205-499; src/cuda/makeplan.cu:223-398,
 // setpts.cu:77-314, execute.cu:105-271) with these differences of form:
 //   * the bin sort is deterministic (stable radix sort, sort.hpp);
-//   * setpts for types 1/2 never synchronizes with the host (the subproblem
-//     table is sized by an upper bound and its true length stays on the GPU);
+//   * setpts for types 1/2 synchronizes with the host only for 3-D fp32
+//     spreading plans (to size the dense-bin launch exactly); otherwise the
+//     subproblem table is sized by an upper bound and its true length stays
+//     on the GPU;
 //   * the fine-grid work buffer is allocated once at plan time, not at every
 //     execute, and grid clearing uses a kernel rather than a memset node, so
 //     execute is capturable in a hipGraph;
@@ -26,7 +28,12 @@
 namespace fa {
 
 template <typename T>
+class BatchPlan;
+
+template <typename T>
 class Plan {
+  friend class BatchPlan<T>;
+
  public:
   using C = cplx_t<T>;
 
@@ -41,6 +48,9 @@ class Plan {
   // the opposite sign and vice versa (the reference CPU library's
   // execute_adjoint, include/finufft/execute.hpp:318-357). Types 1 and 2 only.
   void execute_adjoint(C *c, C *fk);
+  // Type-2 direction of the plan with the derivatives of the result with
+  // respect to every point coordinate: c (may be null) and dc[dim][ntransf][M].
+  void interp_grad(C *c, C *dc, C *fk);
 
   // introspection (tests, bench)
   int type() const { return type_; }
@@ -105,6 +115,7 @@ class Plan {
   dbuf<T> sxyz_[3];  // coordinates in sorted order
   bool colored_ = false;
   int sub_capacity_ = 0;
+  int sub_size_ = 1024;  // points per subproblem
   int dense_min_ = 0;    // column spreading: bins above this count use subproblems
   int n_dense_sub_ = 0;  // column spreading: number of those subproblems
   SortScratch scratch_;
@@ -113,6 +124,7 @@ class Plan {
   // fine grid work buffer and FFT
   dbuf<C> fw_;
   FftPlan fft_;
+  dbuf<C> grad_scratch_;  // interp_grad when ntransf exceeds the batch size
 
   // type 3
   dbuf<T> xyzp_[3], stup_[3];

@@ -107,6 +107,60 @@ __global__ void amplify_to_grid_kernel(ModeMap mm, const T *k0, const T *k1, con
 }
 
 template <typename T>
+__global__ void __launch_bounds__(EW_T)
+    amplify_batch_kernel(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
+                         cplx_t<T> *const *fw_base, int nprob, int64_t fk_vec_offset,
+                         int64_t total_blocks) {
+  const int64_t b = blockIdx.x;
+  if (b >= total_blocks) return;
+  // The block's problem: the last item whose first block is <= b.
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].blk_begin <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const AmpItem<T> &it = items[lo];
+  const int64_t local = b - it.blk_begin;
+  const int t = int(local / it.chunks);
+  const int64_t g0 = (local % it.chunks) * int64_t(AMP_CHUNK);
+  const cplx_t<T> *fk = fk_base[lo] + fk_vec_offset * it.nmodes + int64_t(t) * it.nmodes;
+  cplx_t<T> *fw = fw_base[lo] + int64_t(t) * it.nftot;
+  const int64_t g1 = g0 + AMP_CHUNK < it.nftot ? g0 + AMP_CHUNK : it.nftot;
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += EW_T) {
+    int64_t rem = g, kidx = 0, kstride = 1;
+    T kv = T(1);
+    bool inside = true;
+    for (int d = 0; d < it.dim; ++d) {
+      const int gd = int(rem % it.nf[d]);
+      rem /= it.nf[d];
+      const int m = it.ms[d];
+      int fr;
+      if (gd < m - m / 2)
+        fr = gd;
+      else if (gd >= it.nf[d] - m / 2)
+        fr = gd - it.nf[d];
+      else {
+        inside = false;
+        break;
+      }
+      const int k = it.modeord == 0 ? fr + m / 2 : (fr >= 0 ? fr : m + fr);
+      kv *= it.ker[d][fr >= 0 ? fr : -fr];
+      kidx += k * kstride;
+      kstride *= m;
+    }
+    cplx_t<T> out{0, 0};
+    if (inside) {
+      const cplx_t<T> v = fk[kidx];
+      out = cplx_t<T>{v.x / kv, v.y / kv};
+    }
+    fw[g] = out;
+  }
+}
+
+template <typename T>
 __global__ void zero_kernel(cplx_t<T> *p, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
@@ -235,6 +289,18 @@ void deconvolve(int dim, int modeord, bool to_modes, const int ms[3], const int 
 }
 
 template <typename T>
+void amplify_batch(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
+                   cplx_t<T> *const *fw_base, int nprob, int nvec, int64_t fk_vec_offset,
+                   int64_t total_blocks, hipStream_t stream) {
+  (void)nvec;
+  if (total_blocks <= 0 || nprob <= 0) return;
+  if (total_blocks > INT32_MAX) fail(FINUFFT_ERR_MAXNALLOC, "amplify_batch: grid too large");
+  amplify_batch_kernel<T><<<unsigned(total_blocks), EW_T, 0, stream>>>(
+      items, fk_base, fw_base, nprob, fk_vec_offset, total_blocks);
+  launch_check("amplify_batch_kernel");
+}
+
+template <typename T>
 void zero(cplx_t<T> *p, int64_t n, hipStream_t stream) {
   if (n <= 0) return;
   zero_kernel<T><<<ew_grid(n), EW_T, 0, stream>>>(p, n);
@@ -304,6 +370,8 @@ void cmul(const cplx_t<T> *a, const cplx_t<T> *b, cplx_t<T> *out, int n, int blk
   template void deconvolve<T>(int, int, bool, const int[3], const int[3], const T *const[3],  \
                               cplx_t<T> *, cplx_t<T> *, int, hipStream_t);                     \
   template void zero<T>(cplx_t<T> *, int64_t, hipStream_t);                                    \
+  template void amplify_batch<T>(const AmpItem<T> *, const cplx_t<T> *const *,                 \
+                                 cplx_t<T> *const *, int, int, int64_t, int64_t, hipStream_t); \
   template void minmax<T>(const T *, int, double *, double *, hipStream_t);                    \
   template void affine<T>(const T *, T *, int, double, double, hipStream_t);                   \
   template void t3_prephase<T>(int, const T *const[3], const double[3], int, int, cplx_t<T> *, \

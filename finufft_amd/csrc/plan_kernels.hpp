@@ -27,6 +27,30 @@ void deconvolve(int dim, int modeord, bool to_modes, const int ms[3], const int 
                 const T *const ker[3], cplx_t<T> *fw, cplx_t<T> *fk, int blk,
                 hipStream_t stream);
 
+// One problem of a fused amplification launch (many independent type-2
+// problems): modes fk (vector 0 of the round, nmodes per vector) -> fine grid
+// fw (nftot per vector), for nvec vectors. blk_begin is the first block of
+// this problem in the launch; every problem gets nvec * chunks blocks of
+// AMP_CHUNK cells.
+template <typename T>
+struct AmpItem {
+  int dim, modeord;
+  int ms[3], nf[3];
+  const T *ker[3];
+  int64_t nmodes, nftot;
+  int64_t blk_begin;
+  int chunks;  // blocks per vector
+};
+constexpr int AMP_CHUNK = 4096;
+
+// Amplifies all problems of a round in one launch. fk_base[p] / fw_base[p]
+// are the problem's vector-0 pointers of the round; total_blocks is the sum
+// of nvec * chunks. Writes every fine-grid cell (zeros outside the modes).
+template <typename T>
+void amplify_batch(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
+                   cplx_t<T> *const *fw_base, int nprob, int nvec, int64_t fk_vec_offset,
+                   int64_t total_blocks, hipStream_t stream);
+
 // Sets n complex values to zero (graph-capture friendly, no memset node).
 template <typename T>
 void zero(cplx_t<T> *p, int64_t n, hipStream_t stream);

@@ -212,6 +212,10 @@ __global__ void bin_key_kernel(int M, const T *__restrict__ x, const T *__restri
       const int nsx = g.binsize[0] / g.colS, nsy = g.binsize[1] / g.colS;
       const int sub = min(l[0] / g.colS, nsx - 1) + nsx * min(l[1] / g.colS, nsy - 1);
       key = (key * (nsx * nsy) + sub) * g.binsize[2] + l[2];
+    } else if (g.bank) {
+      const int P0 = g.binsize[0] + g.ns - 1, P1 = g.binsize[1] + g.ns - 1;
+      const int cell = l[0] + P0 * (DIM > 1 ? l[1] + P1 * (DIM > 2 ? l[2] : 0) : 0);
+      key = key * 32 + (cell & 31);
     }
     keys[i] = key;
     return;
@@ -305,10 +309,11 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
               SortScratch &s, hipStream_t stream, int *colstart) {
   const int nb = g.total();
   const bool cols = g.colS > 0 && g.dim == 3 && g.start_bins;
+  const bool banks = !cols && g.bank && g.start_bins;
   if (cols && (colstart == nullptr || g.nkeys() > INT32_MAX))
     fail(FINUFFT_ERR_INVALID_ARGUMENT, "bin_sort: invalid sub-column geometry");
   const int nsub = cols ? g.nsub() : 1;
-  if (M == 0 || (nb == 1 && !cols)) {
+  if (M == 0 || (nb == 1 && !cols && !banks)) {
     iota(idx, M, stream);
     two_point_bounds_kernel<<<blocks_for(nb + 1, 256), 256, 0, stream>>>(binstart, nb, M);
     launch_check("two_point_bounds_kernel");
@@ -339,7 +344,8 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
   }
   launch_check("bin_key_kernel");
 
-  const int64_t nkeys = cols ? g.nkeys() : nb;
+  const int64_t nkeys = (cols || banks) ? g.nkeys() : nb;
+  if (nkeys > INT32_MAX) fail(FINUFFT_ERR_INVALID_ARGUMENT, "bin_sort: too many keys");
   const int B = std::max(1, 32 - __builtin_clz(unsigned(nkeys - 1)));  // bits of the largest key
   const int passes = (B + 7) / 8;
   const int bpp = (B + passes - 1) / passes;
@@ -370,8 +376,8 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
                                                                                binstart);
     launch_check("strided_copy_kernel");
   } else {
-    bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(kin, M, nb, 1,
-                                                                          binstart);
+    bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(
+        kin, M, nb, banks ? 32 : 1, binstart);
     launch_check("bin_bounds_kernel");
   }
 }

@@ -33,9 +33,17 @@ struct BinGeom {
   // first-node positions in x and y over the bin's whole z extent. Points of a
   // sub-column are then contiguous and ordered by z (ties by index).
   int colS = 0;
+  // bank = 1 (start bins, no sub-columns) refines the key inside each bin by
+  // the LDS bank class of the point's first tile cell: (lx + P0 ly +
+  // P0 P1 lz) mod 32 with P = binsize + ns - 1. The interpolation kernel deals
+  // a bin's points to lanes with a stride, so that each half-wave reads 32
+  // different bank classes.
+  int bank = 0;
   int total() const { return nbins[0] * nbins[1] * nbins[2]; }
   int nsub() const { return colS ? (binsize[0] / colS) * (binsize[1] / colS) : 1; }
-  int64_t nkeys() const { return int64_t(total()) * nsub() * (colS ? binsize[2] : 1); }
+  int64_t nkeys() const {
+    return int64_t(total()) * nsub() * (colS ? binsize[2] : 1) * (bank && !colS ? 32 : 1);
+  }
 };
 
 // Scratch reused across setpts calls (grows only).

@@ -678,28 +678,25 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
   }
 }
 
-// Whole-bin interpolation: stage the padded tile, then gather per point.
+// Bin interpolation: stage the padded tile, then gather per point. The unit is
+// a whole bin, or (interp_sub) a subproblem of at most maxsub points of one
+// bin, so that a dense bin is shared by several workgroups.
+// Interpolates one unit (bin or subproblem) of vector grid g into cout.
 template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(BIN_THREADS)
-    interp_bin_kernel(SpreadCtx<T> ctx, cplx_t<T> *__restrict__ c,
-                      const cplx_t<T> *__restrict__ fw, int64_t nftot) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  T *re = reinterpret_cast<T *>(lds_raw);
-  StartTile<DIM, NS> st;
-  st.init_bin(ctx, blockIdx.x);
-  if (st.npts == 0) return;
-  T *im = re + st.ntile;
-  const int t = blockIdx.y;
-  const cplx_t<T> *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
-    const cplx_t<T> v = g[st.grid_index(ctx, n)];
-    re[n] = v.x;
-    im[n] = v.y;
-  }
+__device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const StartTile<DIM, NS> &st,
+                                            cplx_t<T> *tile, const cplx_t<T> *__restrict__ g,
+                                            cplx_t<T> *__restrict__ cout) {
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) tile[n] = g[st.grid_index(ctx, n)];
   __syncthreads();
-  cplx_t<T> *cout = c + int64_t(t) * ctx.M;
-  for (int i = threadIdx.x; i < st.npts; i += BIN_THREADS) {
-    const int q = st.first + i;
+  // Strided dealing: work item i = 32 g + m takes the unit's point m G + g
+  // (G = ceil(npts / 32)). When the bin is ordered by LDS bank class (the
+  // sort's bank refinement), the 32 lanes of a half-wave then start their
+  // gathers in 32 different bank classes.
+  const int G = (st.npts + 31) >> 5;
+  for (int i = threadIdx.x; i < 32 * G; i += BIN_THREADS) {
+    const int pos = (i & 31) * G + (i >> 5);
+    if (pos >= st.npts) continue;
+    const int q = st.first + pos;
     const int j = ctx.idx[q];
     T ker[DIM][NS];
     int l[DIM];
@@ -711,17 +708,19 @@ __global__ void __launch_bounds__(BIN_THREADS)
     if constexpr (DIM == 1) {
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
-        ar = std::fma(re[l[0] + a], ker[0][a], ar);
-        ai = std::fma(im[l[0] + a], ker[0][a], ai);
+        const cplx_t<T> v = tile[l[0] + a];
+        ar = std::fma(v.x, ker[0][a], ar);
+        ai = std::fma(v.y, ker[0][a], ai);
       }
     } else if constexpr (DIM == 2) {
       for (int b = 0; b < NS; ++b) {
-        const int row = (l[1] + b) * st.P[0] + l[0];
+        const cplx_t<T> *row = tile + (l[1] + b) * st.P[0] + l[0];
         T rr = 0, ri = 0;
 #pragma unroll
         for (int a = 0; a < NS; ++a) {
-          rr = std::fma(re[row + a], ker[0][a], rr);
-          ri = std::fma(im[row + a], ker[0][a], ri);
+          const cplx_t<T> v = row[a];
+          rr = std::fma(v.x, ker[0][a], rr);
+          ri = std::fma(v.y, ker[0][a], ri);
         }
         ar = std::fma(rr, ker[1][b], ar);
         ai = std::fma(ri, ker[1][b], ai);
@@ -730,12 +729,13 @@ __global__ void __launch_bounds__(BIN_THREADS)
       for (int e = 0; e < NS; ++e) {
         T pr = 0, pi = 0;
         for (int b = 0; b < NS; ++b) {
-          const int row = ((l[2] + e) * st.P[1] + l[1] + b) * st.P[0] + l[0];
+          const cplx_t<T> *row = tile + ((l[2] + e) * st.P[1] + l[1] + b) * st.P[0] + l[0];
           T rr = 0, ri = 0;
 #pragma unroll
           for (int a = 0; a < NS; ++a) {
-            rr = std::fma(re[row + a], ker[0][a], rr);
-            ri = std::fma(im[row + a], ker[0][a], ri);
+            const cplx_t<T> v = row[a];
+            rr = std::fma(v.x, ker[0][a], rr);
+            ri = std::fma(v.y, ker[0][a], ri);
           }
           pr = std::fma(rr, ker[1][b], pr);
           pi = std::fma(ri, ker[1][b], pi);
@@ -745,6 +745,149 @@ __global__ void __launch_bounds__(BIN_THREADS)
       }
     }
     cout[j] = cplx_t<T>{ar, ai};
+  }
+}
+
+// Bin interpolation: stage the padded tile, then gather per point. The unit is
+// a whole bin, or (interp_sub) a subproblem of at most maxsub points of one
+// bin, so that a dense bin is shared by several workgroups. The tile holds
+// interleaved complex values: one 8-byte (fp32) or 16-byte (fp64) LDS read
+// per cell.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(BIN_THREADS)
+    interp_bin_kernel(SpreadCtx<T> ctx, cplx_t<T> *__restrict__ c,
+                      const cplx_t<T> *__restrict__ fw, int64_t nftot) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cplx_t<T> *tile = reinterpret_cast<cplx_t<T> *>(lds_raw);
+  StartTile<DIM, NS> st;
+  if (ctx.interp_sub) {
+    if (!st.init_sub(ctx, blockIdx.x)) return;
+  } else {
+    st.init_bin(ctx, blockIdx.x);
+  }
+  if (st.npts == 0) return;
+  const int t = blockIdx.y;
+  interp_unit<T, DIM, NS>(ctx, st, tile, fw + int64_t(t) * nftot, c + int64_t(t) * ctx.M);
+}
+
+// Persistent interpolation of many problems (see interp_batch): each
+// workgroup walks work units blockIdx.x, blockIdx.x + gridDim.x, ...
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(BIN_THREADS)
+    interp_batch_kernel(const InterpItem<T> *__restrict__ items, int nprob,
+                        const cplx_t<T> *const *__restrict__ fw_base,
+                        cplx_t<T> *const *__restrict__ c_base, int64_t c_vec_offset,
+                        int64_t total_units) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cplx_t<T> *tile = reinterpret_cast<cplx_t<T> *>(lds_raw);
+  for (int64_t u = blockIdx.x; u < total_units; u += gridDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].unit_begin <= u)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const InterpItem<T> &it = items[lo];
+    const int64_t local = u - it.unit_begin;
+    const int t = int(local / it.nsub);
+    const int s = int(local % it.nsub);
+    StartTile<DIM, NS> st;
+    const bool ok = st.init_sub(it.ctx, s);
+    if (ok && st.npts > 0) {
+      interp_unit<T, DIM, NS>(it.ctx, st, tile, fw_base[lo] + int64_t(t) * it.nftot,
+                              c_base[lo] + (c_vec_offset + t) * int64_t(it.ctx.M));
+    }
+    __syncthreads();  // the tile is reused by the next unit
+  }
+}
+
+// Interpolant and coordinate derivatives, one lane per point (sorted order),
+// reading the fine grid from HBM: c = sum g kx ky kz and
+// dc/dx_d = sum g (d k_d / dx_d) prod_{e != d} k_e, where d k / dx =
+// (d k / d xl) (-nf / 2 pi). Used by the backward pass; the extra FMAs share
+// the grid reads of the plain interpolation.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(SPREAD_THREADS)
+    interp_grad_kernel(SpreadCtx<T> ctx, cplx_t<T> *__restrict__ c, cplx_t<T> *__restrict__ dc,
+                       const cplx_t<T> *__restrict__ fw, int64_t nftot, int blk) {
+  const int i = blockIdx.x * SPREAD_THREADS + threadIdx.x;
+  if (i >= ctx.M) return;
+  const int j = ctx.idx[i];
+  T ker[3][NS], dker[3][NS];
+  int st[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    st[d] = start_and_kernel_d<T, NS>(coord(ctx, d, i, j), ctx.nf[d], ker[d], dker[d], ctx.kp);
+    if (st[d] < 0) st[d] += ctx.nf[d];
+    const T scale = T(-double(ctx.nf[d]) / (2.0 * 3.14159265358979323846));
+#pragma unroll
+    for (int a = 0; a < NS; ++a) dker[d][a] *= scale;
+  }
+  int xi[NS];
+#pragma unroll
+  for (int a = 0; a < NS; ++a) xi[a] = wrap_once(st[0] + a, ctx.nf[0]);
+  const int64_t mb = int64_t(ctx.M) * blk;
+  for (int t = 0; t < blk; ++t) {
+    const cplx_t<T> *g = fw + int64_t(t) * nftot;
+    // v: value, d0..d2: derivatives (re, im)
+    T v[2] = {0, 0}, d0[2] = {0, 0}, d1[2] = {0, 0}, d2[2] = {0, 0};
+    const int ne = DIM > 2 ? NS : 1, nb = DIM > 1 ? NS : 1;
+    for (int e = 0; e < ne; ++e) {
+      const int iz = DIM > 2 ? wrap_once(st[2] + e, ctx.nf[2]) : 0;
+      T pv[2] = {0, 0}, p0[2] = {0, 0}, p1[2] = {0, 0};
+      for (int b = 0; b < nb; ++b) {
+        const int iy = DIM > 1 ? wrap_once(st[1] + b, ctx.nf[1]) : 0;
+        const cplx_t<T> *row = g + (int64_t(iz) * ctx.nf[1] + iy) * ctx.nf[0];
+        T rv[2] = {0, 0}, r0[2] = {0, 0};
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+          const cplx_t<T> w = row[xi[a]];
+          rv[0] = std::fma(w.x, ker[0][a], rv[0]);
+          rv[1] = std::fma(w.y, ker[0][a], rv[1]);
+          r0[0] = std::fma(w.x, dker[0][a], r0[0]);
+          r0[1] = std::fma(w.y, dker[0][a], r0[1]);
+        }
+        if constexpr (DIM > 1) {
+          const T ky = ker[1][b], dky = dker[1][b];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            pv[u] = std::fma(rv[u], ky, pv[u]);
+            p0[u] = std::fma(r0[u], ky, p0[u]);
+            p1[u] = std::fma(rv[u], dky, p1[u]);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            pv[u] = rv[u];
+            p0[u] = r0[u];
+          }
+        }
+      }
+      if constexpr (DIM > 2) {
+        const T kz = ker[2][e], dkz = dker[2][e];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          v[u] = std::fma(pv[u], kz, v[u]);
+          d0[u] = std::fma(p0[u], kz, d0[u]);
+          d1[u] = std::fma(p1[u], kz, d1[u]);
+          d2[u] = std::fma(pv[u], dkz, d2[u]);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          v[u] = pv[u];
+          d0[u] = p0[u];
+          d1[u] = p1[u];
+        }
+      }
+    }
+    const int64_t o = int64_t(t) * ctx.M + j;
+    if (c) c[o] = cplx_t<T>{v[0], v[1]};
+    dc[o] = cplx_t<T>{d0[0], d0[1]};
+    if constexpr (DIM > 1) dc[mb + o] = cplx_t<T>{d1[0], d1[1]};
+    if constexpr (DIM > 2) dc[2 * mb + o] = cplx_t<T>{d2[0], d2[1]};
   }
 }
 
@@ -833,10 +976,35 @@ void interp_ns(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int b
   } else {
     auto k = interp_bin_kernel<T, DIM, NS>;
     allow_lds(k, ctx.lds_bytes);
-    dim3 grid(ctx.nbins[0] * ctx.nbins[1] * ctx.nbins[2], blk);
+    dim3 grid(ctx.interp_sub ? ctx.sub_capacity : ctx.nbins[0] * ctx.nbins[1] * ctx.nbins[2],
+              blk);
     k<<<grid, BIN_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
     launch_check("interp_bin_kernel");
   }
+}
+
+template <typename T, int DIM, int NS>
+void interp_batch_ns(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *fw_base,
+                     cplx_t<T> *const *c_base, int64_t c_vec_offset, int64_t total_units,
+                     size_t lds_bytes, hipStream_t stream) {
+  if (total_units <= 0) return;
+  auto k = interp_batch_kernel<T, DIM, NS>;
+  allow_lds(k, lds_bytes);
+  // persistent grid: enough workgroups to fill the GPU several times over
+  const int64_t g = std::min<int64_t>(total_units, 256 * 16);
+  k<<<unsigned(g), BIN_THREADS, lds_bytes, stream>>>(items, nprob, fw_base, c_base, c_vec_offset,
+                                                     total_units);
+  launch_check("interp_batch_kernel");
+}
+
+template <typename T, int DIM, int NS>
+void interp_grad_ns(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
+                    int blk, hipStream_t stream) {
+  if (ctx.M == 0) return;
+  dim3 grid((ctx.M + SPREAD_THREADS - 1) / SPREAD_THREADS);
+  interp_grad_kernel<T, DIM, NS><<<grid, SPREAD_THREADS, 0, stream>>>(ctx, c, dc, fw,
+                                                                      grid_total(ctx), blk);
+  launch_check("interp_grad_kernel");
 }
 
 // Compile-time width dispatch over [2, max width of the precision].
@@ -869,7 +1037,40 @@ struct InterpOp {
   }
 };
 
+template <typename T, int DIM, int NS>
+struct InterpBatchOp {
+  static void go(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *fw_base,
+                 cplx_t<T> *const *c_base, int64_t c_vec_offset, int64_t total_units,
+                 size_t lds_bytes, hipStream_t s) {
+    interp_batch_ns<T, DIM, NS>(items, nprob, fw_base, c_base, c_vec_offset, total_units,
+                                lds_bytes, s);
+  }
+};
+
+template <typename T, int DIM, int NS>
+struct InterpGradOp {
+  static void go(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
+                 int blk, hipStream_t s) {
+    interp_grad_ns<T, DIM, NS>(ctx, c, dc, fw, blk, s);
+  }
+};
+
 }  // namespace
+
+template <typename T, int DIM>
+void interp_grad_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
+                     int blk, hipStream_t stream) {
+  ns_dispatch<T, InterpGradOp>::run(ctx.ns, ctx, c, dc, fw, blk, stream);
+}
+
+template <typename T, int DIM>
+void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns,
+                      const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
+                      int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                      hipStream_t stream) {
+  ns_dispatch<T, InterpBatchOp>::run(ns, items, nprob, fw_base, c_base, c_vec_offset,
+                                     total_units, lds_bytes, stream);
+}
 
 template <typename T, int DIM>
 void spread_dim(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
@@ -887,5 +1088,12 @@ template void spread_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, const cplx
                                           cplx_t<FA_REAL> *, int, hipStream_t);
 template void interp_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, cplx_t<FA_REAL> *,
                                           const cplx_t<FA_REAL> *, int, hipStream_t);
+template void interp_batch_dim<FA_REAL, FA_DIM>(const InterpItem<FA_REAL> *, int, int,
+                                                const cplx_t<FA_REAL> *const *,
+                                                cplx_t<FA_REAL> *const *, int64_t, int64_t,
+                                                size_t, hipStream_t);
+template void interp_grad_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, cplx_t<FA_REAL> *,
+                                               cplx_t<FA_REAL> *, const cplx_t<FA_REAL> *, int,
+                                               hipStream_t);
 
 }  // namespace fa

@@ -61,6 +61,9 @@ struct SpreadCtx {
   // (nbins_tot * nsub + 1 entries). Colored launches skip bins with more than
   // dense_min points; those bins are covered by n_dense_sub subproblems
   // (substart / sub2bin built over the dense bins only) with an atomic flush.
+  // Interpolation by subproblems (substart / sub2bin cover every bin).
+  int interp_sub = 0;
+
   int colS = 0;
   const int *colstart = nullptr;
   int dense_min = 0;
@@ -90,6 +93,33 @@ template <typename T>
 void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
             hipStream_t stream);
 
+// fw (nf_tot x blk) -> c (M x blk, may be null) and dc (dim x blk x M): the
+// interpolant and its derivatives with respect to each point coordinate,
+// through the derivative of the kernel.
+template <typename T>
+void interp_grad(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
+                 int blk, hipStream_t stream);
+
+// One problem of a fused interpolation launch over many independent type-2
+// problems (bin subproblems of each problem, nvec vectors per round).
+template <typename T>
+struct InterpItem {
+  SpreadCtx<T> ctx;   // interp_sub = 1: substart / sub2bin cover every bin
+  int64_t nftot;      // fine-grid cells per vector
+  int nsub;           // subproblems of the problem (work units per vector)
+  int64_t unit_begin;  // first work unit of this problem in the launch
+};
+
+// Interpolates all problems of a round in one persistent launch: work unit u
+// of problem p is (vector u / nsub, subproblem u % nsub); fw_base[p] is the
+// problem's vector-0 grid of the round, c_base[p] its value array (vector
+// c_vec_offset is the round's first). All problems share the kernel width ns.
+template <typename T>
+void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns,
+                  const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
+                  int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                  hipStream_t stream);
+
 // Per-dimension entry points (one translation unit each).
 template <typename T, int DIM>
 void spread_dim(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
@@ -97,6 +127,14 @@ void spread_dim(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int 
 template <typename T, int DIM>
 void interp_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
                 hipStream_t stream);
+template <typename T, int DIM>
+void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns,
+                      const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
+                      int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                      hipStream_t stream);
+template <typename T, int DIM>
+void interp_grad_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
+                     int blk, hipStream_t stream);
 
 
 }  // namespace fa

@@ -23,6 +23,67 @@ FINUFFT_EXPORT int cufinufft_execute_adjoint(cufinufft_plan d_plan, hipDoubleCom
 FINUFFT_EXPORT int cufinufftf_execute_adjoint(cufinufftf_plan d_plan, hipFloatComplex *d_c,
                                               hipFloatComplex *d_fk);
 
+/* The type-2 direction of a type 1 or 2 plan (the transform of a type-2
+ * plan, the adjoint of a type-1 plan) on the modes fk, with the derivatives
+ * of its output with respect to every point coordinate through the
+ * derivative of the spreading kernel: c[ntransf][M] (may be NULL) and
+ * dc[dim][ntransf][M], dimension 0 = x. No reference counterpart (the
+ * reference has no gradient code); it is the coordinate part of the backward
+ * pass of both transform types. */
+FINUFFT_EXPORT int cufinufft_interp_grad(cufinufft_plan d_plan, hipDoubleComplex *d_c,
+                                         hipDoubleComplex *d_dc, hipDoubleComplex *d_fk);
+FINUFFT_EXPORT int cufinufftf_interp_grad(cufinufftf_plan d_plan, hipFloatComplex *d_c,
+                                          hipFloatComplex *d_dc, hipFloatComplex *d_fk);
+
+/* Fused execution of many independent type-2 problems. No reference
+ * counterpart: the reference runs one plan per problem. plans[nplans] are
+ * type-2 plans (cufinufft_plan or cufinufftf_plan as is_double says) with
+ * points set, sharing dimension, ntransf, isign and kernel width, made with
+ * gpu_method 0 or 2. arena_bytes bounds the fine-grid memory the batch
+ * allocates (0: 16 GiB). The plans must outlive the batch and keep their
+ * points. execute takes host arrays of nplans device pointers: c[p] receives
+ * the ntransf x M_p values of problem p, fk[p] holds its ntransf x N_p modes.
+ * The whole schedule is recorded into a HIP graph on the first execute and
+ * replayed afterwards, on the stream of plans[0]. */
+typedef void *finufft_amd_batch;
+FINUFFT_EXPORT int finufft_amd_batch_makeplan(int is_double, int nplans, void *const *plans,
+                                              size_t arena_bytes, finufft_amd_batch *out);
+FINUFFT_EXPORT int finufft_amd_batch_execute(finufft_amd_batch batch, void *const *c,
+                                             void *const *fk);
+FINUFFT_EXPORT int finufft_amd_batch_destroy(finufft_amd_batch batch);
+/* iout[4]: number of problems, vectors per round, distinct fine-grid shapes,
+ * 1 when the schedule runs as a recorded graph. use_graph = 0 runs direct
+ * launches instead (for comparison). */
+FINUFFT_EXPORT int finufft_amd_batch_info(finufft_amd_batch batch, int64_t *iout);
+FINUFFT_EXPORT int finufft_amd_batch_set_graph(finufft_amd_batch batch, int use_graph);
+
+/* The learned cost model's network (no reference counterpart; the
+ * reference picks methods from fixed tables, src/cuda/heuristics.cu:270-280).
+ * A multilayer perceptron with FINUFFT_AMD_MLP_IN inputs, two ReLU layers of
+ * FINUFFT_AMD_MLP_HIDDEN units and FINUFFT_AMD_MLP_OUT linear outputs; the
+ * parameters are one packed fp32 array W1[H][IN] b1[H] W2[H][H] b2[H]
+ * W3[OUT][H] b3[OUT]. All arrays are device arrays, samples row-major;
+ * stream is a hipStream_t (NULL: the null stream).
+ *   forward: y[n][OUT] from x[n][IN];
+ *   grad:    adds to grad[NPARAM] and *loss the gradient and value of
+ *            mean_b 0.5 sum_j mask[b][j] (y[b][j] - target[b][j])^2 (grad and
+ *            loss must be zeroed by the caller);
+ *   adam:    one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at step >= 1. */
+#define FINUFFT_AMD_MLP_IN 8
+#define FINUFFT_AMD_MLP_HIDDEN 32
+#define FINUFFT_AMD_MLP_OUT 4
+#define FINUFFT_AMD_MLP_NPARAM                                                       \
+  (FINUFFT_AMD_MLP_HIDDEN * FINUFFT_AMD_MLP_IN + FINUFFT_AMD_MLP_HIDDEN +            \
+   FINUFFT_AMD_MLP_HIDDEN * FINUFFT_AMD_MLP_HIDDEN + FINUFFT_AMD_MLP_HIDDEN +        \
+   FINUFFT_AMD_MLP_OUT * FINUFFT_AMD_MLP_HIDDEN + FINUFFT_AMD_MLP_OUT)
+FINUFFT_EXPORT int finufft_amd_mlp_forward(const float *params, int n, const float *x, float *y,
+                                           void *stream);
+FINUFFT_EXPORT int finufft_amd_mlp_grad(const float *params, int n, const float *x,
+                                        const float *target, const float *mask, float *grad,
+                                        float *loss, void *stream);
+FINUFFT_EXPORT int finufft_amd_mlp_adam(float *params, const float *grad, float *m1, float *m2,
+                                        int step, float lr, void *stream);
+
 /* Plan introspection; no reference counterpart (the reference exposes these
  * values only through opts.debug printing, src/cuda/makeplan.cu:116-118).
  * iout[16]: type, dim, ns, nf1, nf2, nf3, gpu_method, binsize x/y/z,

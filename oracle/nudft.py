@@ -103,6 +103,22 @@ def type2(pts, f, isign, modeord=0, idx=None):
     return out if batched else out[0]
 
 
+def type2_grad(pts, f, isign, modeord=0):
+    """Derivatives of the type-2 values with respect to the point coordinates:
+    out[d] = sum_k f_k (i isign k_d) exp(i isign k . x_j), shape (dim, [ntr,] M)
+    (the closed-form derivative of the direct sum)."""
+    dim = len(pts)
+    f = np.asarray(f, np.complex128)
+    n_modes = f.shape[-dim:]
+    out = []
+    for d in range(dim):
+        shape = [1] * f.ndim
+        shape[f.ndim - dim + d] = n_modes[d]
+        kd = freqs(n_modes[d], modeord).astype(np.float64).reshape(shape)
+        out.append(type2(pts, f * (1j * isign * kd), isign, modeord))
+    return np.stack(out)
+
+
 def type3(pts, c, isign, targets, idx=None):
     """f at targets (or targets[idx]); c: (M,) or (ntr, M)."""
     c = np.asarray(c, np.complex128)

@@ -52,11 +52,13 @@ def start_nodes(pts_c, nf, ns, fold64=True):
     return out
 
 
-def start_bin_keys(pts_c, nf, binsize, ns, fold64=True, col_side=0):
+def start_bin_keys(pts_c, nf, binsize, ns, fold64=True, col_side=0, bank=False):
     """Bins of the first support node. With col_side (3-D), the key is refined
     inside each bin to (sub-column, local z): sub-columns are col_side x
     col_side first-node squares in (x, y), x fastest, and points of a
-    sub-column are ordered by the local z of their first node."""
+    sub-column are ordered by the local z of their first node. With bank, the
+    key is refined by (lx + P0 ly + P0 P1 lz) mod 32, P = binsize + ns - 1 (the
+    LDS bank class of the point's first tile cell)."""
     nb = nbins(nf, binsize)
     key = np.zeros(len(pts_c[0]), np.int64)
     loc = []
@@ -70,19 +72,27 @@ def start_bin_keys(pts_c, nf, binsize, ns, fold64=True, col_side=0):
         nsx, nsy = binsize[0] // col_side, binsize[1] // col_side
         sub = np.minimum(loc[0] // col_side, nsx - 1) + nsx * np.minimum(loc[1] // col_side, nsy - 1)
         key = (key * (nsx * nsy) + sub) * binsize[2] + loc[2]
+    elif bank:
+        P = [b + ns - 1 for b in binsize]
+        cell = loc[0].copy()
+        if len(loc) > 1:
+            inner = loc[1] + (P[1] * loc[2] if len(loc) > 2 else 0)
+            cell = cell + P[0] * inner
+        key = key * 32 + (cell & 31)
     return key
 
 
-def bin_sort(pts_c, nf, binsize, start_ns=None, fold64=True, col_side=0):
+def bin_sort(pts_c, nf, binsize, start_ns=None, fold64=True, col_side=0, bank=False):
     """(permutation, binstart) of the stable bin sort; start_ns selects bins of
     the first support node for kernel width start_ns, col_side the sub-column
-    refinement of those bins (3-D)."""
+    refinement of those bins (3-D), bank the bank-class refinement."""
     if start_ns is None:
         key = bin_keys(pts_c, nf, binsize)
         bin_key = key
     else:
-        key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64, col_side)
-        bin_key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64) if col_side else key
+        key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64, col_side, bank)
+        refined = col_side or bank
+        bin_key = start_bin_keys(pts_c, nf, binsize, start_ns, fold64) if refined else key
     perm = np.argsort(key, kind="stable").astype(np.int32)
     total = int(np.prod(nbins(nf, binsize)))
     counts = np.bincount(bin_key, minlength=total)

@@ -1,0 +1,109 @@
+"""Fused execution of many uneven type-2 problems (finufft_amd.batch).
+
+Every problem of the batch must give what its own plan gives (same kernels;
+the FFT runs batched over more vectors, so agreement is to rounding), and the
+direct sum to the reference's 25 tol criterion. The recorded graph is
+replayed on new data and compared with direct launches.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import nudft
+from tests.helpers import cnormal, rng, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(torch, nprob, ntr, dtype, seed, dim=2):
+    from finufft_amd import Plan
+
+    r = rng(seed)
+    real = np.float32 if dtype == "complex64" else np.float64
+    tol = 1e-5 if dtype == "complex64" else 1e-11
+    plans, pts, fks = [], [], []
+    for _ in range(nprob):
+        n = int(np.exp(r.uniform(np.log(8), np.log(48))))
+        M = int(np.exp(r.uniform(np.log(100), np.log(3000))))
+        shape = (n,) * dim
+        x = [r.uniform(-np.pi, np.pi, M).astype(real) for _ in range(dim)]
+        p = Plan(2, shape, n_trans=ntr, eps=tol, dtype=dtype)
+        xd = [to_dev(torch, a) for a in x]
+        p.setpts(*xd)
+        plans.append(p)
+        pts.append((x, xd))
+        fks.append(cnormal(r, (ntr,) + shape if ntr > 1 else shape, np.dtype(dtype)))
+    return plans, pts, fks, tol
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batch_matches_plans_and_direct(torch_gpu, dtype, dim):
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    ntr = 4
+    plans, pts, fks, tol = _problems(torch, 12, ntr, dtype, 80 + dim, dim)
+    fk_d = [to_dev(torch, f) for f in fks]
+    outs = [torch.empty((ntr, len(x[0])), dtype=fk.dtype, device="cuda")
+            for (x, _), fk in zip(pts, fk_d)]
+    b = BatchPlan(plans, arena_bytes=1 << 26)  # small arena: several rounds
+    info = b.info()
+    assert info["nprob"] == 12 and info["vectors_per_round"] <= ntr
+    b.execute(fk_d, outs)
+    torch.cuda.synchronize()
+    for p, (x, _), fk, fkh, o in zip(plans, pts, fk_d, fks, outs):
+        ref_plan = to_host(p.execute(fk))
+        got = to_host(o)
+        assert nudft.relerr(got, ref_plan) < 50 * np.finfo(got.real.dtype).eps * 10
+        ref = nudft.type2([a.astype(np.float64) for a in x], fkh, p._isign)
+        assert nudft.relerr(got, ref) < 25 * tol
+    print(info)
+
+
+def test_batch_graph_replay(torch_gpu):
+    """Replays with new inputs and outputs, graph against direct launches."""
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    ntr = 2
+    plans, pts, fks, tol = _problems(torch, 20, ntr, "complex64", 90)
+    b = BatchPlan(plans)
+    r = rng(91)
+    results = []
+    for rep in range(3):
+        fk = [to_dev(torch, cnormal(r, f.shape, np.dtype("complex64"))) for f in fks]
+        out_g = [torch.empty((ntr, len(x[0])), dtype=torch.complex64, device="cuda")
+                 for x, _ in pts]
+        out_e = [torch.empty_like(o) for o in out_g]
+        b.set_graph(True)
+        b.execute(fk, out_g)
+        torch.cuda.synchronize()
+        b.set_graph(False)
+        b.execute(fk, out_e)
+        torch.cuda.synchronize()
+        for a, e in zip(out_g, out_e):
+            assert torch.equal(a, e)
+        results.append([to_host(o) for o in out_g])
+    assert b.info()["graph"] == 1
+    # different inputs gave different outputs (the pointer table is refreshed)
+    assert not np.allclose(results[0][0], results[1][0])
+
+
+def test_batch_rejects_mixed_plans(torch_gpu):
+    from finufft_amd import FinufftError, Plan
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    r = rng(92)
+    x = [to_dev(torch, r.uniform(-np.pi, np.pi, 100).astype(np.float32)) for _ in range(2)]
+    p2 = Plan(2, (16, 16), eps=1e-5, dtype="complex64")
+    p2.setpts(*x)
+    p1 = Plan(1, (16, 16), eps=1e-5, dtype="complex64")
+    p1.setpts(*x)
+    p3 = Plan(2, (16, 16), eps=1e-5, dtype="complex64", n_trans=2)
+    p3.setpts(*x)
+    with pytest.raises(FinufftError):
+        BatchPlan([p2, p1])
+    with pytest.raises(FinufftError):
+        BatchPlan([p2, p3])

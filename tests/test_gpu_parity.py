@@ -168,8 +168,8 @@ def test_sort_permutation_bit_exact(torch_gpu):
     # edge coordinates: the box ends, far outside, and just below +pi
     pts[0][:6] = np.array([-np.pi, np.pi, 3 * np.pi, -7.5, np.nextafter(np.float32(np.pi), 0),
                            0.0], np.float32)
-    for method in (1, 2):
-        plan = Plan(1, (64, 48, 40), eps=1e-6, dtype="complex64", gpu_method=method)
+    for ttype, method in ((1, 1), (1, 2), (2, 2)):
+        plan = Plan(ttype, (64, 48, 40), eps=1e-6, dtype="complex64", gpu_method=method)
         plan.setpts(*[to_dev(torch, p) for p in pts])
         info = plan.info()
         perm = torch.empty(M, dtype=torch.int32, device="cuda")
@@ -179,12 +179,14 @@ def test_sort_permutation_bit_exact(torch_gpu):
         assert ier == 0
         pts_c = pts[::-1]  # x-fastest C order
         # method 1: the reference's bins of the folded coordinate; the LDS
-        # method: bins of the first support node
+        # method: bins of the first support node, refined by sub-column
+        # (3-D fp32 spreading) or by LDS bank class (interpolation)
         ref_perm, ref_start = osort.bin_sort(pts_c, list(info["nf"]), list(info["binsize"]),
                                              start_ns=None if method == 1 else info["ns"],
-                                             col_side=info["col_side"])
-        assert np.array_equal(to_host(perm), ref_perm), f"method {method}"
-        assert np.array_equal(to_host(binstart), ref_start), f"method {method}"
+                                             col_side=info["col_side"],
+                                             bank=(ttype == 2 and method == 2))
+        assert np.array_equal(to_host(perm), ref_perm), f"type {ttype} method {method}"
+        assert np.array_equal(to_host(binstart), ref_start), f"type {ttype} method {method}"
 
 
 @pytest.mark.parametrize("shape", [(1024,), (40, 80), (16, 16, 16), (16, 24, 32)])
@@ -308,3 +310,30 @@ def test_column_spread_user_bins(torch_gpu):
     ref_perm, _ = osort.bin_sort(pts[::-1], list(info["nf"]), [8, 8, 4], start_ns=info["ns"],
                                  col_side=2)
     assert np.array_equal(to_host(perm), ref_perm)
+
+
+@pytest.mark.parametrize("maxsub", [1024, 16])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_interp_dense_bins(torch_gpu, maxsub, dtype):
+    """Interpolation by subproblems: a tight cluster puts thousands of points
+    in one bin, which is then shared by several workgroups."""
+    from finufft_amd import Plan
+
+    torch = torch_gpu
+    r = rng(25)
+    shape = (24, 24, 24)
+    M = 12000
+    real = REAL[dtype]
+    pts = uniform_pts(r, 3, M, real)
+    x0 = 7 * 2 * np.pi / 48
+    for d in range(3):
+        pts[d][:10000] = (x0 + 0.02 * r.standard_normal(10000)).astype(real)
+    f = cnormal(r, shape, np.dtype(dtype))
+    tol = TOL[dtype]
+    plan = Plan(2, shape, eps=tol, dtype=dtype, gpu_maxsubprobsize=maxsub)
+    plan.setpts(*[to_dev(torch, p) for p in pts])
+    c = to_host(plan.execute(to_dev(torch, f)))
+    ref = nudft.type2([p.astype(np.float64) for p in pts], f, plan._isign)
+    err = nudft.relerr(c, ref)
+    print(maxsub, dtype, plan.info(), err)
+    assert err < 25 * tol

@@ -352,3 +352,74 @@ def test_fp32_width_table():
                                             ctypes.byref(ns), ctypes.byref(beta)) == 0
         assert ns.value == want, tol
         assert ns.value == kernel.kernel_shape(tol, 2.0, is_double=False)[0]
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_bank_sort_vs_bruteforce(dim):
+    """Bank-class refinement of the start bins (interpolation plans): bins in
+    x-fastest order, inside a bin increasing (lx + P0 ly + P0 P1 lz) mod 32,
+    ties by index; walked literally."""
+    r = np.random.default_rng(7 + dim)
+    M = 300
+    pts = [r.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(dim)]
+    nf = [32, 24, 16][:dim]
+    bs = [16, 8, 8][:dim]
+    ns = 7
+    perm, start = osort.bin_sort(pts, nf, bs, start_ns=ns, bank=True)
+    s = osort.start_nodes(pts, nf, ns)
+    P = [b + ns - 1 for b in bs]
+    nb = [n // b for n, b in zip(nf, bs)]
+    bins = np.zeros(M, np.int64)
+    cls = np.zeros(M, np.int64)
+    for i in range(M):
+        b, stride, cell, cstride = 0, 1, 0, 1
+        for d in range(dim):
+            b += (s[d][i] // bs[d]) * stride
+            stride *= nb[d]
+            cell += (s[d][i] % bs[d]) * cstride
+            cstride *= P[d]
+        bins[i], cls[i] = b, cell % 32
+    order = [i for b in range(int(np.prod(nb))) for k in range(32) for i in range(M)
+             if bins[i] == b and cls[i] == k]
+    assert np.array_equal(perm, np.array(order, np.int32))
+    _, start0 = osort.bin_sort(pts, nf, bs, start_ns=ns)
+    assert np.array_equal(start, start0)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_type2_grad_closed_form_vs_fd(dim):
+    """The oracle's coordinate derivative of the type-2 direct sum against
+    central differences of the direct sum itself (FP64, h = 1e-6)."""
+    r = np.random.default_rng(30 + dim)
+    M = 40
+    shape = (9, 8, 7)[:dim]
+    pts = [r.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+    f = r.standard_normal(shape) + 1j * r.standard_normal(shape)
+    g = nudft.type2_grad(pts, f, -1)
+    h = 1e-6
+    for d in range(dim):
+        up = [p.copy() for p in pts]
+        dn = [p.copy() for p in pts]
+        up[d] += h
+        dn[d] -= h
+        fd = (nudft.type2(up, f, -1) - nudft.type2(dn, f, -1)) / (2 * h)
+        assert nudft.relerr(g[d], fd) < 1e-7
+
+
+def test_mlp_oracle_grad_vs_fd():
+    """The FP64 cost-model network gradient against central differences of
+    its own loss (a dropped term or a transposed weight fails this)."""
+    from oracle import mlp
+
+    r = np.random.default_rng(11)
+    p = r.normal(0, 0.3, mlp.NPARAM)
+    x = r.normal(0, 1, (20, mlp.IN))
+    t = r.normal(0, 1, (20, mlp.OUT))
+    m = (r.uniform(size=(20, mlp.OUT)) > 0.3).astype(float)
+    g = mlp.grad(p, x, t, m)
+    h = 1e-6
+    for i in r.choice(mlp.NPARAM, 60, replace=False):
+        e = np.zeros(mlp.NPARAM)
+        e[i] = h
+        fd = (mlp.loss(p + e, x, t, m) - mlp.loss(p - e, x, t, m)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), i
