@@ -6,12 +6,18 @@ Workload (the headline configuration): 3-D type 1 and type 2, fp32
 [-pi, pi)^3, N = 256^3 modes, complex N(0,1) strengths / coefficients,
 isign +1 (type 1) and -1 (type 2). Inputs are generated on the GPU (seeded)
 and resident in HBM before timing. Plans use the library's default options
-(the reference's method heuristic: LDS subproblems for type 1, point-driven
-for type 2).
+(LDS methods for both types: the column-window spreader for type 1, bank-
+ordered subproblem interpolation for type 2).
 
 One step = one type-1 execute + one type-2 execute over the same points.
 value = 2 M / (time per step) NU pts/s. setpts (the sort) is timed separately
 and reported, not counted.
+
+The second part of the metric, the uneven small-problem comparison
+(`batched`), runs 2000 independent 2-D type-2 problems (N log-uniform in
+[32, 512], M log-uniform in [1e3, 1e5], 32 vectors each, fp32 tol 1e-5) as a
+loop of per-plan executes and as one fused batch replayed from a HIP graph,
+and reports the speedup at the errors measured for both.
 
 Extra fields:
   roofline      the dominant kernel (longest average launch among spread and
@@ -170,6 +176,54 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
     evals = float(np.sum(Ms)) * ntr
     log(f"batched workload: fused {t_batch * 1e3:.1f} ms ({info})")
     del b
+
+    # Learned options: the cost model picks the upsampling factor of every
+    # problem (among the options the fused executor runs); problems of equal
+    # choice form one batch each.
+    learned = None
+    from finufft_amd import costmodel
+
+    model = costmodel.default_model()
+    if model is not None:
+        choices = model.choose_many([(2, (int(n), int(n)), int(m), ntr, tol, "complex64")
+                                     for n, m in zip(Ns, Ms)], methods=(2,))
+        del plans
+        torch.cuda.empty_cache()
+        lplans = []
+        for i, (n, m) in enumerate(zip(Ns, Ms)):
+            opts = choices[i] or {}
+            p = Plan(2, (int(n), int(n)), n_trans=ntr, eps=tol, dtype="complex64", isign=-1,
+                     **opts)
+            p.setpts(*pts[i])
+            lplans.append(p)
+        groups = {}
+        for i, o in enumerate(choices):
+            groups.setdefault(o.get("upsampfac", 0.0), []).append(i)
+        batches = [(BatchPlan([lplans[i] for i in idx]), idx) for idx in groups.values()]
+        for o in outs:
+            o.zero_()
+
+        def run_learned():
+            for bb, idx in batches:
+                bb.execute([fks[i] for i in idx], [outs[i] for i in idx])
+
+        run_learned()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run_learned()
+        torch.cuda.synchronize()
+        t_learned = (time.perf_counter() - t0) / reps
+        e_learned = errors()
+        learned = {
+            "speedup": t_plain / t_learned,
+            "ms": round(t_learned * 1e3, 3),
+            "nu_evals_per_s": evals / t_learned,
+            "error": e_learned,
+            "choices": {str(k): len(v) for k, v in groups.items()},
+        }
+        log(f"batched workload: fused + learned {t_learned * 1e3:.1f} ms ({learned['choices']})")
+        del batches, lplans
     return {
         "workload": (f"{nprob} 2D type-2 problems, N log-uniform [32,512], M log-uniform "
                      f"[1e3,1e5], ntransf {ntr}, fp32 tol {tol:g}"),
@@ -185,6 +239,7 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
         "vectors_per_round": info["vectors_per_round"],
         "grid_shapes": info["grid_shapes"],
         "graph": info["graph"],
+        "learned": learned,
     }
 
 

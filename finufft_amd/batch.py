@@ -53,11 +53,12 @@ class BatchPlan:
             raise FinufftError("finufft_amd_batch_set_graph", ier)
 
     def info(self) -> dict:
-        v = (c_int64 * 4)()
+        v = (c_int64 * 6)()
         ier = _lib.lib().finufft_amd_batch_info(self._h, v)
         if ier != 0:
             raise FinufftError("finufft_amd_batch_info", ier)
-        return {"nprob": v[0], "vectors_per_round": v[1], "grid_shapes": v[2], "graph": v[3]}
+        return {"nprob": v[0], "vectors_per_round": v[1], "grid_shapes": v[2], "graph": v[3],
+                "capture_stage": v[4], "capture_error": v[5]}
 
     def __del__(self):
         if getattr(self, "_h", None):

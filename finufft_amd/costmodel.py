@@ -178,17 +178,27 @@ class CostModel:
         span = self.hi - self.lo
         return bool(np.all(f >= self.lo - 0.1 * span) and np.all(f <= self.hi + 0.1 * span))
 
-    def choose(self, ttype, n_modes, M, ntransf, tol, dtype):
+    def choose(self, ttype, n_modes, M, ntransf, tol, dtype, methods=(1, 2)):
         """Plan options (a dict for Plan(**opts)) of the predicted fastest
-        allowed option, or {} (the library's heuristic) when the features are
-        outside the training range."""
-        f = features(ttype, n_modes, M, ntransf, tol, dtype)
-        if not self.in_range(f):
-            return {}
-        pred = self.predict(f)[0]
-        banned = set(self.rejected.get(problem_class(ttype, len(n_modes), tol, dtype), []))
-        order = [i for i in np.argsort(pred) if int(i) not in banned]
-        return dict(ARMS[int(order[0])]) if order else {}
+        allowed option among those with gpu_method in `methods`, or {} (the
+        library's heuristic) when the features are outside the training
+        range."""
+        return self.choose_many([(ttype, n_modes, M, ntransf, tol, dtype)], methods)[0]
+
+    def choose_many(self, problems, methods=(1, 2)):
+        """choose() for many problems with one inference launch."""
+        feats = np.stack([features(*p) for p in problems])
+        pred = self.predict(feats)
+        out = []
+        for f, y, (ttype, n_modes, M, ntransf, tol, dtype) in zip(feats, pred, problems):
+            if not self.in_range(f):
+                out.append({})
+                continue
+            banned = set(self.rejected.get(problem_class(ttype, len(n_modes), tol, dtype), []))
+            order = [int(i) for i in np.argsort(y)
+                     if int(i) not in banned and ARMS[int(i)]["gpu_method"] in methods]
+            out.append(dict(ARMS[order[0]]) if order else {})
+        return out
 
 
 _default = None

@@ -128,23 +128,33 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStr
             "hipMemcpy(batch grids)");
   fk_host_.resize(n);
   c_host_.resize(n);
+  // Raise the interpolation kernel's LDS limit now, outside any capture.
+  interp_batch<T>(interp_items_.get(), n, dim_, ns_, fw_base_.get(), c_base_.get(), 0, 0,
+                  lds_bytes_, stream_);
+  hip_check(hipStreamCreateWithFlags(&own_, hipStreamNonBlocking), "hipStreamCreate(batch)");
+  hip_check(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming), "hipEventCreate(batch)");
+  hip_check(hipEventCreateWithFlags(&ev_out_, hipEventDisableTiming), "hipEventCreate(batch)");
 }
 
 template <typename T>
 BatchPlan<T>::~BatchPlan() {
+  if (own_) (void)hipStreamSynchronize(own_);
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
   if (graph_) (void)hipGraphDestroy(graph_);
+  if (ev_in_) (void)hipEventDestroy(ev_in_);
+  if (ev_out_) (void)hipEventDestroy(ev_out_);
+  if (own_) (void)hipStreamDestroy(own_);
 }
 
 template <typename T>
-void BatchPlan<T>::enqueue() {
+void BatchPlan<T>::enqueue(hipStream_t s) {
   const int n = nprob();
   for (int v0 = 0; v0 < ntr_; v0 += V_) {
     amplify_batch<T>(amp_items_.get(), fk_base_.get(), fw_base_.get(), n, V_, v0, amp_blocks_,
-                     stream_);
-    for (auto &g : groups_) g->fft.exec(arena_.get() + g->off, iflag_, stream_);
+                     s);
+    for (auto &g : groups_) g->fft.exec(arena_.get() + g->off, iflag_, s);
     interp_batch<T>(interp_items_.get(), n, dim_, ns_, fw_base_.get(), c_base_.get(), v0,
-                    interp_units_, lds_bytes_, stream_);
+                    interp_units_, lds_bytes_, s);
   }
 }
 
@@ -158,28 +168,45 @@ void BatchPlan<T>::execute(C *const *c, C *const *fk) {
     c_host_[i] = c[i];
     fk_host_[i] = fk[i];
   }
-  // Synchronous copies of the small pointer tables: the host vectors may be
-  // rewritten by the next call at once.
+  // The previous execute may still read the pointer tables: wait for it, then
+  // copy them synchronously (the host vectors may be rewritten at once).
+  hip_check(hipEventSynchronize(ev_out_), "hipEventSynchronize(batch)");
   hip_check(hipMemcpy(fk_base_.get(), fk_host_.data(), n * sizeof(C *), hipMemcpyHostToDevice),
             "hipMemcpy(batch inputs)");
   hip_check(hipMemcpy(c_base_.get(), c_host_.data(), n * sizeof(C *), hipMemcpyHostToDevice),
             "hipMemcpy(batch outputs)");
   if (use_graph_ && !graph_failed_ && gexec_ == nullptr) {
     // Record the whole schedule once; a failed capture falls back to direct
-    // launches for the life of the batch.
-    bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    // launches for the life of the batch (the reason is kept for info()).
+    auto failed = [&](int stage, hipError_t e) {
+      if (capture_stage_ == 0) {
+        capture_stage_ = stage;
+        capture_error_ = int(e);
+      }
+    };
+    hipError_t e = hipStreamBeginCapture(own_, hipStreamCaptureModeThreadLocal);
+    bool ok = e == hipSuccess;
+    if (!ok) failed(1, e);
     if (ok) {
       try {
-        enqueue();
+        enqueue(own_);
       } catch (const nufft_error &) {
         ok = false;
+        failed(2, hipGetLastError());
       }
       hipGraph_t g = nullptr;
-      const bool ended = hipStreamEndCapture(stream_, &g) == hipSuccess;
-      ok = ok && ended && g != nullptr;
-      if (ok && hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0) != hipSuccess) {
-        gexec_ = nullptr;
+      e = hipStreamEndCapture(own_, &g);
+      if (e != hipSuccess || g == nullptr) {
+        if (ok) failed(3, e);
         ok = false;
+      }
+      if (ok) {
+        e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          gexec_ = nullptr;
+          ok = false;
+          failed(4, e);
+        }
       }
       if (ok) {
         graph_ = g;
@@ -192,11 +219,15 @@ void BatchPlan<T>::execute(C *const *c, C *const *fk) {
       graph_failed_ = true;
     }
   }
+  hip_check(hipEventRecord(ev_in_, stream_), "hipEventRecord(batch in)");
+  hip_check(hipStreamWaitEvent(own_, ev_in_, 0), "hipStreamWaitEvent(batch in)");
   if (gexec_ && use_graph_) {
-    hip_check(hipGraphLaunch(gexec_, stream_), "hipGraphLaunch(batch)");
+    hip_check(hipGraphLaunch(gexec_, own_), "hipGraphLaunch(batch)");
   } else {
-    enqueue();
+    enqueue(own_);
   }
+  hip_check(hipEventRecord(ev_out_, own_), "hipEventRecord(batch out)");
+  hip_check(hipStreamWaitEvent(stream_, ev_out_, 0), "hipStreamWaitEvent(batch out)");
 }
 
 template class BatchPlan<float>;

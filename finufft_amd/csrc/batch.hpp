@@ -9,7 +9,8 @@
 //   * one batched rocFFT call per distinct fine-grid shape (problems of equal
 //     shape are contiguous in a shared arena),
 //   * one persistent interpolation launch over every problem's subproblems,
-// all recorded once into a hipGraph and replayed on every execute. The
+// all recorded once into a hipGraph and replayed on every execute (on a
+// stream of the batch's own, ordered with the caller's stream by events). The
 // per-problem input/output pointers live in a small device table that is
 // refreshed before each replay, so the graph never changes.
 #pragma once
@@ -44,6 +45,11 @@ class BatchPlan {
   int vectors_per_round() const { return V_; }
   int groups() const { return int(groups_.size()); }
   bool graph() const { return gexec_ != nullptr; }
+  // Why the schedule could not be recorded (0: it was, or not tried yet):
+  // capture stage (1 begin, 2 a call inside, 3 end, 4 instantiate) and the
+  // HIP error code of that stage.
+  int capture_stage() const { return capture_stage_; }
+  int capture_error() const { return capture_error_; }
   void set_use_graph(bool on) { use_graph_ = on; }
 
  private:
@@ -53,10 +59,14 @@ class BatchPlan {
     int64_t off = 0;  // cells
     FftPlan fft;
   };
-  void enqueue();
+  void enqueue(hipStream_t s);
 
   std::vector<Plan<T> *> plans_;
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr;  // the plans' (caller's) stream
+  // The schedule runs on a stream of its own (the legacy null stream cannot
+  // be captured), ordered after and before the caller's stream by events.
+  hipStream_t own_ = nullptr;
+  hipEvent_t ev_in_ = nullptr, ev_out_ = nullptr;
   int dim_ = 0, ntr_ = 1, iflag_ = 1, ns_ = 2, V_ = 1;
   size_t lds_bytes_ = 0;
   dbuf<C> arena_;
@@ -71,6 +81,7 @@ class BatchPlan {
   std::vector<C *> c_host_;
   bool use_graph_ = true;
   bool graph_failed_ = false;
+  int capture_stage_ = 0, capture_error_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t gexec_ = nullptr;
 };

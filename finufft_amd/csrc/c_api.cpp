@@ -239,6 +239,8 @@ int finufft_amd_batch_info(finufft_amd_batch batch, int64_t *iout) {
       iout[1] = b.vectors_per_round();
       iout[2] = b.groups();
       iout[3] = b.graph() ? 1 : 0;
+      iout[4] = b.capture_stage();
+      iout[5] = b.capture_error();
     };
     if (h->is_double)
       fill(*h->d);

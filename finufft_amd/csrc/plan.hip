@@ -333,6 +333,7 @@ void Plan<T>::setup_bins() {
     geom_.start_bins = 0;
     colored_ = false;
     lds_bytes_ = 0;
+    lds_spread_ = 0;
   };
   geom_.colS = 0;
   geom_.bank = 0;
@@ -357,7 +358,8 @@ void Plan<T>::setup_bins() {
       geom_.start_bins = 1;
       geom_.colS = fa::col_side(ns_);
       colored_ = colr;
-      lds_bytes_ = size_t(tile_cells(3, bs, ns_)) * 16 + col_stage_bytes();
+      lds_spread_ = size_t(tile_cells(3, bs, ns_)) * 16 + col_stage_bytes();
+      lds_bytes_ = size_t(tile_cells(3, bs, ns_)) * 2 * sizeof(T);
     }
   }
   if (col_done) {
@@ -392,11 +394,13 @@ void Plan<T>::setup_bins() {
     } else {
       geom_.start_bins = 1;
       colored_ = spreads && colorable(dim_, nf_, bs, ns_);
-      lds_bytes_ = size_t(tile_cells(dim_, bs, ns_)) * cell;
+      lds_bytes_ = size_t(tile_cells(dim_, bs, ns_)) * 2 * sizeof(T);
+      lds_spread_ = size_t(tile_cells(dim_, bs, ns_)) * 2 * sizeof(double);
       // interpolation plans order each bin by LDS bank class
       geom_.bank = spreads ? 0 : 1;
     }
   }
+  spread_gm_ = lds_spread_ > size_t(lds_limit);
   for (int d = 0; d < 3; ++d)
     if (bs[d] < 1) fail(FINUFFT_ERR_BINSIZE_NOTVALID, "invalid bin size");
   if (opts_.gpu_maxsubprobsize < 1) opts_.gpu_maxsubprobsize = 1024;
@@ -602,6 +606,8 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.sub2bin = sub2bin_.get();
   c.sub_capacity = sub_capacity_;
   c.lds_bytes = lds_bytes_;
+  c.lds_spread = lds_spread_;
+  c.spread_gm = spread_gm_ ? 1 : 0;
   c.colored = colored_ ? 1 : 0;
   c.colS = geom_.colS;
   c.colstart = colstart_.get();

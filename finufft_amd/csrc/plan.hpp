@@ -119,7 +119,9 @@ class Plan {
   int dense_min_ = 0;    // column spreading: bins above this count use subproblems
   int n_dense_sub_ = 0;  // column spreading: number of those subproblems
   SortScratch scratch_;
-  size_t lds_bytes_ = 0;
+  size_t lds_bytes_ = 0;   // interpolation tile
+  size_t lds_spread_ = 0;  // spreading tile (fp64) plus staging
+  bool spread_gm_ = false;  // spreading tile too large: point-driven spreading
 
   // fine grid work buffer and FFT
   dbuf<C> fw_;

@@ -17,6 +17,8 @@
 #include "spreadinterp.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -898,12 +900,20 @@ int64_t grid_total(const SpreadCtx<T> &ctx) {
   return int64_t(ctx.nf[0]) * ctx.nf[1] * ctx.nf[2];
 }
 
+// Raises a kernel's dynamic LDS limit once per (kernel, size); later launches
+// (including those recorded into a graph) make no runtime call for it.
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
-  if (bytes > 48 * 1024)
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)),
-              "hipFuncSetAttribute(LDS)");
+  if (bytes <= 48 * 1024) return;
+  static std::mutex mu;
+  static std::map<const void *, size_t> done;
+  const void *key = reinterpret_cast<const void *>(kernel);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = done.find(key);
+  if (it != done.end() && it->second >= bytes) return;
+  hip_check(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)),
+            "hipFuncSetAttribute(LDS)");
+  done[key] = bytes;
 }
 
 template <typename T, int DIM, int NS>
@@ -911,7 +921,7 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
                hipStream_t stream) {
   if (ctx.M == 0) return;
   const int64_t nftot = grid_total(ctx);
-  if (ctx.method == 1) {
+  if (ctx.method == 1 || ctx.spread_gm) {
     dim3 grid((ctx.M + SPREAD_THREADS - 1) / SPREAD_THREADS, blk);
     spread_gm_kernel<T, DIM, NS><<<grid, SPREAD_THREADS, 0, stream>>>(ctx, c, fw, nftot);
     launch_check("spread_gm_kernel");
@@ -919,21 +929,21 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
     if constexpr (DIM == 3 && std::is_same<T, float>::value && NS <= COL_MAX_NS) {
       if (ctx.colored) {
         auto k = spread_col_colored_kernel<NS>;
-        allow_lds(k, ctx.lds_bytes);
+        allow_lds(k, ctx.lds_spread);
         for (int color = 0; color < 8; ++color) {
           int col[3] = {color & 1, (color >> 1) & 1, (color >> 2) & 1};
           int64_t nblk = 1;
           for (int d = 0; d < 3; ++d) nblk *= (ctx.nbins[d] - col[d] + 1) / 2;
           if (nblk == 0) continue;
-          k<<<dim3(unsigned(nblk), blk), 64 * COL_WAVES, ctx.lds_bytes, stream>>>(
+          k<<<dim3(unsigned(nblk), blk), 64 * COL_WAVES, ctx.lds_spread, stream>>>(
               ctx, c, fw, nftot, col[0], col[1], col[2]);
           launch_check("spread_col_colored_kernel");
         }
       }
       if (ctx.n_dense_sub > 0) {
         auto k = spread_col_sub_kernel<NS>;
-        allow_lds(k, ctx.lds_bytes);
-        k<<<dim3(unsigned(ctx.n_dense_sub), blk), 64 * COL_WAVES, ctx.lds_bytes, stream>>>(
+        allow_lds(k, ctx.lds_spread);
+        k<<<dim3(unsigned(ctx.n_dense_sub), blk), 64 * COL_WAVES, ctx.lds_spread, stream>>>(
             ctx, c, fw, nftot);
         launch_check("spread_col_sub_kernel");
       }
@@ -942,7 +952,7 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
     }
   } else if (ctx.colored) {
     auto k = spread_bin_colored_kernel<T, DIM, NS>;
-    allow_lds(k, ctx.lds_bytes);
+    allow_lds(k, ctx.lds_spread);
     for (int color = 0; color < (1 << DIM); ++color) {
       int col[3] = {0, 0, 0};
       int64_t nblk = 1;
@@ -951,15 +961,15 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
         nblk *= (ctx.nbins[d] - col[d] + 1) / 2;
       }
       if (nblk == 0) continue;
-      k<<<dim3(unsigned(nblk), blk), BIN_THREADS, ctx.lds_bytes, stream>>>(
+      k<<<dim3(unsigned(nblk), blk), BIN_THREADS, ctx.lds_spread, stream>>>(
           ctx, c, fw, nftot, col[0], col[1], col[2]);
       launch_check("spread_bin_colored_kernel");
     }
   } else {
     auto k = spread_sub_kernel<T, DIM, NS>;
-    allow_lds(k, ctx.lds_bytes);
+    allow_lds(k, ctx.lds_spread);
     dim3 grid(ctx.sub_capacity, blk);
-    k<<<grid, BIN_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
+    k<<<grid, BIN_THREADS, ctx.lds_spread, stream>>>(ctx, c, fw, nftot);
     launch_check("spread_sub_kernel");
   }
 }
@@ -987,9 +997,9 @@ template <typename T, int DIM, int NS>
 void interp_batch_ns(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *fw_base,
                      cplx_t<T> *const *c_base, int64_t c_vec_offset, int64_t total_units,
                      size_t lds_bytes, hipStream_t stream) {
-  if (total_units <= 0) return;
   auto k = interp_batch_kernel<T, DIM, NS>;
   allow_lds(k, lds_bytes);
+  if (total_units <= 0) return;
   // persistent grid: enough workgroups to fill the GPU several times over
   const int64_t g = std::min<int64_t>(total_units, 256 * 16);
   k<<<unsigned(g), BIN_THREADS, lds_bytes, stream>>>(items, nprob, fw_base, c_base, c_vec_offset,

@@ -50,7 +50,12 @@ struct SpreadCtx {
   const int *substart = nullptr;  // nbins_tot + 1 entries, last = total count
   const int *sub2bin = nullptr;
   int sub_capacity = 0;  // launch grid (>= the device-side total)
-  size_t lds_bytes = 0;  // LDS tile bytes of the method-2 kernels
+  size_t lds_bytes = 0;   // LDS bytes of the method-2 interpolation kernels
+  size_t lds_spread = 0;  // LDS bytes of the method-2 spreading kernels (fp64 tiles)
+  // Spread point-driven (method-1 kernel) although the plan is method 2: set
+  // when the fp64 spreading tile of the plan's bins exceeds the LDS limit
+  // (interpolation plans, whose bins are sized for plan-precision tiles).
+  int spread_gm = 0;
   // Method-2 spreading by whole bins in 2^dim parity colors with a plain
   // (non-atomic) tile flush; requires per dimension an even number (>= 2) of
   // equal bins of at least 2 ceil(ns/2) + 1 cells.

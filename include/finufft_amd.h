@@ -51,9 +51,11 @@ FINUFFT_EXPORT int finufft_amd_batch_makeplan(int is_double, int nplans, void *c
 FINUFFT_EXPORT int finufft_amd_batch_execute(finufft_amd_batch batch, void *const *c,
                                              void *const *fk);
 FINUFFT_EXPORT int finufft_amd_batch_destroy(finufft_amd_batch batch);
-/* iout[4]: number of problems, vectors per round, distinct fine-grid shapes,
- * 1 when the schedule runs as a recorded graph. use_graph = 0 runs direct
- * launches instead (for comparison). */
+/* iout[6]: number of problems, vectors per round, distinct fine-grid shapes,
+ * 1 when the schedule runs as a recorded graph, and when recording failed the
+ * stage (1 begin, 2 a call inside the capture, 3 end, 4 instantiate) and its
+ * HIP error code. use_graph = 0 runs direct launches instead (for
+ * comparison). */
 FINUFFT_EXPORT int finufft_amd_batch_info(finufft_amd_batch batch, int64_t *iout);
 FINUFFT_EXPORT int finufft_amd_batch_set_graph(finufft_amd_batch batch, int use_graph);
 

@@ -85,7 +85,8 @@ def test_batch_graph_replay(torch_gpu):
         for a, e in zip(out_g, out_e):
             assert torch.equal(a, e)
         results.append([to_host(o) for o in out_g])
-    assert b.info()["graph"] == 1
+    info = b.info()
+    assert info["graph"] == 1, info
     # different inputs gave different outputs (the pointer table is refreshed)
     assert not np.allclose(results[0][0], results[1][0])
 

@@ -133,25 +133,35 @@ def test_fp32_3d_tol_1e6(torch_gpu, kernel_eps, bound):
             assert err <= bound
 
 
-def test_adjointness(torch_gpu):
-    """<type1(c), f> = <c, type2(f)> for the plan and its adjoint (test/adjointness.cpp)."""
+@pytest.mark.parametrize("ptype", [1, 2])
+@pytest.mark.parametrize("dtype,shape", [("complex128", (16, 18, 20)), ("complex64", (40, 48, 36)),
+                                         ("complex64", (60, 50)), ("complex128", (300,))])
+def test_adjointness(torch_gpu, ptype, dtype, shape):
+    """<type1(c), f> = <c, type2(f)> through a plan and its adjoint
+    (test/adjointness.cpp), for type-1 and type-2 plans: the adjoint of a
+    type-2 plan spreads with the bins chosen for its interpolation."""
     from finufft_amd import Plan
 
     torch = torch_gpu
     r = rng(5)
-    shape = (16, 18, 20)
     M = 3000
-    pts = uniform_pts(r, 3, M, np.float64)
-    plan = Plan(1, shape, eps=1e-12, dtype="complex128")
+    real = REAL[dtype]
+    pts = uniform_pts(r, len(shape), M, real)
+    tol = 1e-12 if dtype == "complex128" else 1e-6
+    plan = Plan(ptype, shape, eps=tol, dtype=dtype)
     plan.setpts(*[to_dev(torch, p) for p in pts])
-    c = cnormal(r, (M,))
-    f = cnormal(r, shape)
-    F = to_host(plan.execute(to_dev(torch, c)))
-    C = to_host(plan.execute_adjoint(to_dev(torch, f)))
-    lhs = np.vdot(f.ravel(), F.ravel())
-    rhs = np.vdot(C, c)
+    c = cnormal(r, (M,), np.dtype(dtype))
+    f = cnormal(r, shape, np.dtype(dtype))
+    if ptype == 1:
+        F = to_host(plan.execute(to_dev(torch, c)))
+        C = to_host(plan.execute_adjoint(to_dev(torch, f)))
+    else:
+        C = to_host(plan.execute(to_dev(torch, f)))
+        F = to_host(plan.execute_adjoint(to_dev(torch, c)))
+    lhs = np.vdot(f.ravel().astype(np.complex128), F.ravel().astype(np.complex128))
+    rhs = np.vdot(C.astype(np.complex128), c.astype(np.complex128))
     denom = np.linalg.norm(f) * np.linalg.norm(F) / np.sqrt(np.prod(shape))
-    assert abs(lhs - rhs) / denom < 1e-10
+    assert abs(lhs - rhs) / denom < (1e-10 if dtype == "complex128" else 2e-5)
 
 
 def test_sort_permutation_bit_exact(torch_gpu):
