@@ -1,10 +1,11 @@
 """Fused execution of many independent type-2 NUFFT problems.
 
 Wraps finufft_amd_batch_* (include/finufft_amd.h): the type-2 plans of many
-problems, each with its own points, modes and sizes, run per round of vectors
-as one amplification launch, one batched FFT per fine-grid shape and one
-persistent interpolation launch, recorded once into a HIP graph and replayed.
-The reference has no counterpart; its way is a loop of per-plan executes.
+problems, each with its own points, modes and sizes, run in cache-sized chunks
+of problems of equal grid shape, each chunk as one amplification launch, one
+batched FFT and one interpolation launch, recorded once into a HIP graph and
+replayed. The reference has no counterpart; its way is a loop of per-plan
+executes.
 """
 
 from __future__ import annotations
@@ -53,12 +54,12 @@ class BatchPlan:
             raise FinufftError("finufft_amd_batch_set_graph", ier)
 
     def info(self) -> dict:
-        v = (c_int64 * 6)()
+        v = (c_int64 * 7)()
         ier = _lib.lib().finufft_amd_batch_info(self._h, v)
         if ier != 0:
             raise FinufftError("finufft_amd_batch_info", ier)
         return {"nprob": v[0], "vectors_per_round": v[1], "grid_shapes": v[2], "graph": v[3],
-                "capture_stage": v[4], "capture_error": v[5]}
+                "capture_stage": v[4], "capture_error": v[5], "chunks": v[6]}
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -8,7 +8,7 @@
 namespace fa {
 
 template <typename T>
-BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStream_t stream)
+BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStream_t stream)
     : plans_(std::move(plans)), stream_(stream) {
   if (plans_.empty()) fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: no plans");
   for (const Plan<T> *p : plans_)
@@ -22,14 +22,18 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStr
     if (p->type_ != 2 || p->spread_only_ || p->opts_.gpu_method == 1)
       fail(FINUFFT_ERR_INVALID_ARGUMENT,
            "batch: every plan must be a full type-2 transform with gpu_method 2");
-    if (p->dim_ != dim_ || p->ntransf_ != ntr_ || p->iflag_ != iflag_ || p->ns_ != ns_)
+    if (p->dim_ != dim_ || p->ntransf_ != ntr_ || p->iflag_ != iflag_ || p->ns_ != ns_ ||
+        p->opts_.upsampfac != p0.opts_.upsampfac ||
+        p->opts_.gpu_kerevalmeth != p0.opts_.gpu_kerevalmeth)
       fail(FINUFFT_ERR_INVALID_ARGUMENT,
-           "batch: plans must share dimension, ntransf, isign and kernel width");
+           "batch: plans must share dimension, ntransf, isign and kernel (width, upsampfac, "
+           "evaluation method)");
     if (p->opts_.gpu_device_id != p0.opts_.gpu_device_id)
       fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: plans must share the device");
   }
   device_guard dg(p0.opts_.gpu_device_id);
   const int n = nprob();
+  kp_ = p0.spread_ctx().kp;  // identical for all plans (same width and upsampfac)
 
   // Subproblem counts stay on the device after setpts; read them once here.
   std::vector<int> nsub(n, 0);
@@ -41,9 +45,8 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStr
     hip_check(hipStreamSynchronize(p.stream_), "hipStreamSynchronize(batch subproblems)");
   }
 
-  // Problems of equal fine-grid shape share one batched FFT.
+  // Problems of equal fine-grid shape form a group (one FFT shape).
   std::map<std::array<int, 3>, int> gid;
-  std::vector<int> group_of(n), pos_in_group(n);
   for (int i = 0; i < n; ++i) {
     const Plan<T> &p = *plans_[i];
     const std::array<int, 3> key{p.nf_[0], p.nf_[1], p.nf_[2]};
@@ -53,67 +56,91 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStr
       groups_.push_back(std::make_unique<Group>());
       for (int d = 0; d < 3; ++d) groups_.back()->nf[d] = key[d];
     }
-    group_of[i] = it->second;
-    pos_in_group[i] = int(groups_[it->second]->probs.size());
     groups_[it->second]->probs.push_back(i);
   }
 
-  // Vectors per round: the largest divisor of ntransf whose grids fit.
-  int64_t cells = 0;
-  for (const Plan<T> *p : plans_) cells += p->nftot_;
+  // Vectors per round: the largest divisor of ntransf for which the largest
+  // problem's grids fit the chunk budget.
+  int64_t max_cells = 0;
+  for (const Plan<T> *p : plans_) max_cells = std::max<int64_t>(max_cells, p->nftot_);
   V_ = 1;
   for (int v = ntr_; v >= 1; --v) {
     if (ntr_ % v) continue;
-    if (size_t(v) * size_t(cells) * sizeof(C) <= arena_bytes || v == 1) {
+    if (size_t(v) * size_t(max_cells) * sizeof(C) <= chunk_bytes || v == 1) {
       V_ = v;
       break;
     }
   }
-  arena_.resize(size_t(V_) * size_t(cells));
-  int64_t off = 0;
-  for (auto &g : groups_) {
-    g->off = off;
-    const int64_t nft = int64_t(g->nf[0]) * g->nf[1] * g->nf[2];
-    off += nft * V_ * int64_t(g->probs.size());
-    g->fft.setup(dim_, g->nf, V_ * int(g->probs.size()), std::is_same<T, double>::value);
-    g->fft.prepare(iflag_);
-  }
 
-  // Work tables.
+  // Chunks: consecutive problems of one group, as many as fit the budget.
+  const bool fp64 = std::is_same<T, double>::value;
+  int64_t arena_cells = 0;
+  for (int gi = 0; gi < int(groups_.size()); ++gi) {
+    Group &g = *groups_[gi];
+    const int64_t nft = int64_t(g.nf[0]) * g.nf[1] * g.nf[2];
+    const int64_t per = int64_t(V_) * nft * int64_t(sizeof(C));
+    g.per_chunk = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(g.probs.size()),
+                                                              int64_t(chunk_bytes) / per)));
+    const int K = int(g.probs.size());
+    const int rest = K % g.per_chunk;
+    g.fft_full.setup(dim_, g.nf, V_ * g.per_chunk, fp64);
+    g.fft_full.prepare(iflag_);
+    if (rest) {
+      g.fft_rest.setup(dim_, g.nf, V_ * rest, fp64);
+      g.fft_rest.prepare(iflag_);
+    }
+    for (int k = 0; k < K; k += g.per_chunk) {
+      Chunk c;
+      c.first = int(slot_prob_.size()) + k;
+      c.count = std::min(g.per_chunk, K - k);
+      c.group = gi;
+      c.full = c.count == g.per_chunk;
+      c.amp_blocks = c.units = 0;
+      chunks_.push_back(c);
+      arena_cells = std::max<int64_t>(arena_cells, int64_t(c.count) * V_ * nft);
+    }
+    for (int i : g.probs) slot_prob_.push_back(i);
+  }
+  arena_.resize(size_t(arena_cells));
+
+  // Work tables in slot order; prefix counts restart at every chunk.
   std::vector<AmpItem<T>> amp(n);
   std::vector<InterpItem<T>> itp(n);
   std::vector<C *> fwb(n);
-  int64_t blocks = 0, units = 0;
-  for (int i = 0; i < n; ++i) {
-    const Plan<T> &p = *plans_[i];
-    AmpItem<T> &a = amp[i];
-    a.dim = dim_;
-    a.modeord = p.opts_.modeord;
-    for (int d = 0; d < 3; ++d) {
-      a.ms[d] = p.ms_[d];
-      a.nf[d] = p.nf_[d];
-      a.ker[d] = p.fwker_[d < dim_ ? d : 0].get();
+  for (Chunk &c : chunks_) {
+    int64_t blocks = 0, units = 0;
+    for (int s = c.first; s < c.first + c.count; ++s) {
+      const int i = slot_prob_[s];
+      const Plan<T> &p = *plans_[i];
+      AmpItem<T> &a = amp[s];
+      a.dim = dim_;
+      a.modeord = p.opts_.modeord;
+      for (int d = 0; d < 3; ++d) {
+        a.ms[d] = p.ms_[d];
+        a.nf[d] = p.nf_[d];
+        a.rnf[d] = 1.0 / double(p.nf_[d]);
+        a.ker[d] = p.fwker_[d < dim_ ? d : 0].get();
+      }
+      a.nmodes = p.nmodes_;
+      a.nftot = p.nftot_;
+      a.chunks = int((p.nftot_ + AMP_CHUNK - 1) / AMP_CHUNK);
+      a.blk_begin = blocks;
+      blocks += int64_t(a.chunks) * V_;
+
+      InterpItem<T> &it = itp[s];
+      it.ctx = p.spread_ctx();
+      it.ctx.interp_sub = 1;
+      it.nftot = p.nftot_;
+      it.nsub = nsub[i];
+      it.unit_begin = units;
+      units += nsub[i];
+      lds_bytes_ = std::max(lds_bytes_, p.lds_bytes_);
+
+      fwb[s] = arena_.get() + int64_t(s - c.first) * V_ * p.nftot_;
     }
-    a.nmodes = p.nmodes_;
-    a.nftot = p.nftot_;
-    a.chunks = int((p.nftot_ + AMP_CHUNK - 1) / AMP_CHUNK);
-    a.blk_begin = blocks;
-    blocks += int64_t(a.chunks) * V_;
-
-    InterpItem<T> &it = itp[i];
-    it.ctx = p.spread_ctx();
-    it.ctx.interp_sub = 1;
-    it.nftot = p.nftot_;
-    it.nsub = nsub[i];
-    it.unit_begin = units;
-    units += int64_t(nsub[i]) * V_;
-    lds_bytes_ = std::max(lds_bytes_, p.lds_bytes_);
-
-    const Group &g = *groups_[group_of[i]];
-    fwb[i] = arena_.get() + g.off + int64_t(pos_in_group[i]) * V_ * p.nftot_;
+    c.amp_blocks = blocks;
+    c.units = units;
   }
-  amp_blocks_ = blocks;
-  interp_units_ = units;
   amp_items_.resize(n);
   interp_items_.resize(n);
   fw_base_.resize(n);
@@ -129,8 +156,8 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStr
   fk_host_.resize(n);
   c_host_.resize(n);
   // Raise the interpolation kernel's LDS limit now, outside any capture.
-  interp_batch<T>(interp_items_.get(), n, dim_, ns_, fw_base_.get(), c_base_.get(), 0, 0,
-                  lds_bytes_, stream_);
+  interp_batch<T>(interp_items_.get(), n, dim_, ns_, kp_, fw_base_.get(), c_base_.get(), 0, V_,
+                  0, lds_bytes_, stream_);
   hip_check(hipStreamCreateWithFlags(&own_, hipStreamNonBlocking), "hipStreamCreate(batch)");
   hip_check(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming), "hipEventCreate(batch)");
   hip_check(hipEventCreateWithFlags(&ev_out_, hipEventDisableTiming), "hipEventCreate(batch)");
@@ -148,13 +175,16 @@ BatchPlan<T>::~BatchPlan() {
 
 template <typename T>
 void BatchPlan<T>::enqueue(hipStream_t s) {
-  const int n = nprob();
   for (int v0 = 0; v0 < ntr_; v0 += V_) {
-    amplify_batch<T>(amp_items_.get(), fk_base_.get(), fw_base_.get(), n, V_, v0, amp_blocks_,
-                     s);
-    for (auto &g : groups_) g->fft.exec(arena_.get() + g->off, iflag_, s);
-    interp_batch<T>(interp_items_.get(), n, dim_, ns_, fw_base_.get(), c_base_.get(), v0,
-                    interp_units_, lds_bytes_, s);
+    for (const Chunk &c : chunks_) {
+      Group &g = *groups_[c.group];
+      amplify_batch<T>(amp_items_.get() + c.first, fk_base_.get() + c.first,
+                       fw_base_.get() + c.first, c.count, dim_, v0, c.amp_blocks, s);
+      (c.full ? g.fft_full : g.fft_rest).exec(arena_.get(), iflag_, s);
+      interp_batch<T>(interp_items_.get() + c.first, c.count, dim_, ns_, kp_,
+                      fw_base_.get() + c.first, c_base_.get() + c.first, v0, V_, c.units,
+                      lds_bytes_, s);
+    }
   }
 }
 
@@ -162,11 +192,12 @@ template <typename T>
 void BatchPlan<T>::execute(C *const *c, C *const *fk) {
   device_guard dg(plans_[0]->opts_.gpu_device_id);
   const int n = nprob();
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n; ++i)
     if (c[i] == nullptr || fk[i] == nullptr)
       fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: null data pointer");
-    c_host_[i] = c[i];
-    fk_host_[i] = fk[i];
+  for (int s = 0; s < n; ++s) {
+    c_host_[s] = c[slot_prob_[s]];
+    fk_host_[s] = fk[slot_prob_[s]];
   }
   // The previous execute may still read the pointer tables: wait for it, then
   // copy them synchronously (the host vectors may be rewritten at once).

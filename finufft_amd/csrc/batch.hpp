@@ -2,17 +2,18 @@
 //
 // The reference runs one plan per problem, with one launch per stage and
 // vector batch (src/cuda/execute.cu:105-204); for thousands of small, uneven
-// problems the GPU then spends its time between launches. A BatchPlan takes
-// a set of type-2 plans whose points are already set and runs all of them
-// per round of vectors as
-//   * one amplification launch over every problem (fk -> fine grid),
-//   * one batched rocFFT call per distinct fine-grid shape (problems of equal
-//     shape are contiguous in a shared arena),
-//   * one persistent interpolation launch over every problem's subproblems,
-// all recorded once into a hipGraph and replayed on every execute (on a
-// stream of the batch's own, ordered with the caller's stream by events). The
-// per-problem input/output pointers live in a small device table that is
-// refreshed before each replay, so the graph never changes.
+// problems the GPU then spends much of its time between launches. A BatchPlan
+// takes a set of type-2 plans whose points are already set, groups problems of
+// equal fine-grid shape, and cuts each group into chunks whose grids (for a
+// round of V vectors) fit in a cache-sized budget. Every chunk runs
+//   * one amplification launch over its problems (fk -> fine grid),
+//   * one batched rocFFT call (its problems share the shape),
+//   * one interpolation launch over its problems' subproblems,
+// so that its grids stay in the GPU's last-level cache from stage to stage.
+// The whole schedule is recorded once into a hipGraph and replayed on every
+// execute (on a stream of the batch's own, ordered with the caller's stream by
+// events). The per-problem input/output pointers live in a small device table
+// that is refreshed before each replay, so the graph never changes.
 #pragma once
 
 #include <memory>
@@ -30,9 +31,10 @@ class BatchPlan {
   using C = cplx_t<T>;
 
   // plans: type-2 plans with points set, of equal dimension, ntransf, iflag
-  // and kernel width. arena_bytes bounds the shared fine-grid arena (the
-  // vectors of a problem are processed in rounds of V, V dividing ntransf).
-  BatchPlan(std::vector<Plan<T> *> plans, size_t arena_bytes, hipStream_t stream);
+  // and kernel. chunk_bytes bounds the fine-grid memory of one chunk (the
+  // vectors of a problem are processed in rounds of V, V dividing ntransf,
+  // and a chunk never has fewer than one problem).
+  BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStream_t stream);
   ~BatchPlan();
   BatchPlan(const BatchPlan &) = delete;
   BatchPlan &operator=(const BatchPlan &) = delete;
@@ -44,6 +46,7 @@ class BatchPlan {
   int nprob() const { return int(plans_.size()); }
   int vectors_per_round() const { return V_; }
   int groups() const { return int(groups_.size()); }
+  int chunks() const { return int(chunks_.size()); }
   bool graph() const { return gexec_ != nullptr; }
   // Why the schedule could not be recorded (0: it was, or not tried yet):
   // capture stage (1 begin, 2 a call inside, 3 end, 4 instantiate) and the
@@ -56,8 +59,14 @@ class BatchPlan {
   struct Group {
     int nf[3];
     std::vector<int> probs;
-    int64_t off = 0;  // cells
-    FftPlan fft;
+    int per_chunk = 1;  // problems per full chunk
+    FftPlan fft_full, fft_rest;
+  };
+  struct Chunk {
+    int first, count;  // slots [first, first + count)
+    int group;
+    bool full;
+    int64_t amp_blocks, units;
   };
   void enqueue(hipStream_t s);
 
@@ -69,11 +78,13 @@ class BatchPlan {
   hipEvent_t ev_in_ = nullptr, ev_out_ = nullptr;
   int dim_ = 0, ntr_ = 1, iflag_ = 1, ns_ = 2, V_ = 1;
   size_t lds_bytes_ = 0;
+  KerParams<T> kp_{};
   dbuf<C> arena_;
   std::vector<std::unique_ptr<Group>> groups_;
+  std::vector<Chunk> chunks_;
+  std::vector<int> slot_prob_;  // problem index of each slot (group-major order)
   dbuf<AmpItem<T>> amp_items_;
   dbuf<InterpItem<T>> interp_items_;
-  int64_t amp_blocks_ = 0, interp_units_ = 0;
   dbuf<C *> fw_base_;
   dbuf<const C *> fk_base_;
   dbuf<C *> c_base_;

@@ -110,16 +110,19 @@ struct KerParams {
 // Fills ker[i] = phi at the ns grid offsets for a point whose first grid node
 // sits at xl = s - x_rescaled in [-ns/2, -ns/2 + 1]; z = 2 xl + ns - 1 is the
 // same offset scaled to [-1, 1] (the Horner variable).
-template <typename T, int NS>
-__device__ __forceinline__ void eval_kernel(T *ker, T xl, T z, const KerParams<T> &kp) {
+// coef: the Horner table (kp.coef, or the same pointer with a known address
+// space).
+template <typename T, int NS, typename CP>
+__device__ __forceinline__ void eval_kernel_c(T *ker, T xl, T z, const KerParams<T> &kp,
+                                              CP coef) {
   if (kp.horner) {
     constexpr int NC = horner_nc_sigma2(NS);
 #pragma unroll
-    for (int i = 0; i < NS; ++i) ker[i] = kp.coef[(NC - 1) * NS + i];
+    for (int i = 0; i < NS; ++i) ker[i] = coef[(NC - 1) * NS + i];
 #pragma unroll
     for (int k = NC - 2; k >= 0; --k) {
 #pragma unroll
-      for (int i = 0; i < NS; ++i) ker[i] = std::fma(ker[i], z, kp.coef[k * NS + i]);
+      for (int i = 0; i < NS; ++i) ker[i] = std::fma(ker[i], z, coef[k * NS + i]);
     }
   } else {
 #pragma unroll
@@ -129,6 +132,11 @@ __device__ __forceinline__ void eval_kernel(T *ker, T xl, T z, const KerParams<T
       ker[i] = (zsq < T(1)) ? std::exp(kp.beta * (std::sqrt(T(1) - zsq) - T(1))) : T(0);
     }
   }
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ void eval_kernel(T *ker, T xl, T z, const KerParams<T> &kp) {
+  eval_kernel_c<T, NS>(ker, xl, z, kp, kp.coef);
 }
 
 // Start node and kernel values for one coordinate: s = ceil(x - ns/2)
@@ -156,15 +164,16 @@ __device__ __forceinline__ int start_index(T x, int nf, int ns, int fold64) {
   return int(std::ceil(r - T(ns * 0.5)));
 }
 
-template <typename T, int NS>
-__device__ __forceinline__ int start_and_kernel(T x, int nf, T *ker, const KerParams<T> &kp) {
+template <typename T, int NS, typename CP>
+__device__ __forceinline__ int start_and_kernel_c(T x, int nf, T *ker, const KerParams<T> &kp,
+                                                  CP coef) {
   if constexpr (sizeof(T) == 4) {
     if (kp.fold64) {
       double r = fold_rescale(double(x), nf);
       r = (r >= 0.0 && r < double(nf)) ? r : 0.0;
       const int s = int(std::ceil(r - NS * 0.5));
       const double xl = double(s) - r;
-      eval_kernel<T, NS>(ker, T(xl), T(std::fma(2.0, xl, double(NS - 1))), kp);
+      eval_kernel_c<T, NS>(ker, T(xl), T(std::fma(2.0, xl, double(NS - 1))), kp, coef);
       return s;
     }
   }
@@ -175,8 +184,13 @@ __device__ __forceinline__ int start_and_kernel(T x, int nf, T *ker, const KerPa
   r = (r >= T(0) && r < T(nf)) ? r : T(0);
   const int s = int(std::ceil(r - T(NS * 0.5)));
   const T xl = T(s) - r;
-  eval_kernel<T, NS>(ker, xl, std::fma(T(2), xl, T(NS - 1)), kp);
+  eval_kernel_c<T, NS>(ker, xl, std::fma(T(2), xl, T(NS - 1)), kp, coef);
   return s;
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ int start_and_kernel(T x, int nf, T *ker, const KerParams<T> &kp) {
+  return start_and_kernel_c<T, NS>(x, nf, ker, kp, kp.coef);
 }
 
 // Kernel values and their derivatives with respect to the offset xl: the

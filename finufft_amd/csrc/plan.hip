@@ -35,28 +35,30 @@ void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
 }
 
 template <typename T>
-void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns,
+void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns, const KerParams<T> &kp,
                   const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
-                  int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                  int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
                   hipStream_t stream) {
   switch (dim) {
     case 1:
-      return interp_batch_dim<T, 1>(items, nprob, ns, fw_base, c_base, c_vec_offset, total_units,
-                                    lds_bytes, stream);
+      return interp_batch_dim<T, 1>(items, nprob, ns, kp, fw_base, c_base, c_vec_offset, nvec,
+                                    total_units, lds_bytes, stream);
     case 2:
-      return interp_batch_dim<T, 2>(items, nprob, ns, fw_base, c_base, c_vec_offset, total_units,
-                                    lds_bytes, stream);
+      return interp_batch_dim<T, 2>(items, nprob, ns, kp, fw_base, c_base, c_vec_offset, nvec,
+                                    total_units, lds_bytes, stream);
     default:
-      return interp_batch_dim<T, 3>(items, nprob, ns, fw_base, c_base, c_vec_offset, total_units,
-                                    lds_bytes, stream);
+      return interp_batch_dim<T, 3>(items, nprob, ns, kp, fw_base, c_base, c_vec_offset, nvec,
+                                    total_units, lds_bytes, stream);
   }
 }
 template void interp_batch<float>(const InterpItem<float> *, int, int, int,
+                                  const KerParams<float> &,
                                   const cplx_t<float> *const *, cplx_t<float> *const *, int64_t,
-                                  int64_t, size_t, hipStream_t);
+                                  int, int64_t, size_t, hipStream_t);
 template void interp_batch<double>(const InterpItem<double> *, int, int, int,
+                                   const KerParams<double> &,
                                    const cplx_t<double> *const *, cplx_t<double> *const *,
-                                   int64_t, int64_t, size_t, hipStream_t);
+                                   int64_t, int, int64_t, size_t, hipStream_t);
 
 template <typename T>
 void interp_grad(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
@@ -186,23 +188,43 @@ bool choose_col_bins(const int *nf, int ns, size_t budget, size_t limit, int *ou
   return false;
 }
 
-// 3-D interpolation bins: x-elongated shapes (wide rows spread a wave's
-// gathers over more LDS banks), the first in order of preference whose tile
-// fits `budget` (two workgroups per CU) and that still gives min_units bins.
-bool choose_interp_bins_3d(const int *nf, int ns, size_t cell_bytes, size_t budget,
-                           int64_t min_units, int *out) {
-  static const int pref[][3] = {{32, 8, 8}, {32, 8, 4}, {16, 8, 8}, {16, 8, 4},
-                                {16, 4, 4}, {8, 8, 8},  {8, 4, 4},  {4, 4, 4}};
-  for (const auto &b : pref) {
-    if (b[0] > nf[0] || b[1] > nf[1] || b[2] > nf[2]) continue;
-    if (size_t(tile_cells(3, b, ns)) * cell_bytes > budget) continue;
-    int64_t units = 1;
-    for (int d = 0; d < 3; ++d) units *= (nf[d] + b[d] - 1) / b[d];
-    if (units < min_units) continue;
-    for (int d = 0; d < 3; ++d) out[d] = b[d];
-    return true;
+// 2-D and 3-D interpolation bins: x-elongated shapes (wide rows spread a
+// wave's gathers over more LDS banks) whose padded tile fits `budget` (two
+// workgroups per CU) and is at most max_overhead times the bin (tile loads
+// stay a small multiple of the grid): the first in order of preference that
+// gives min_units bins, else (small grids) the one with the most bins.
+bool choose_interp_bins(int dim, const int *nf, int ns, size_t cell_bytes, size_t budget,
+                        int64_t min_units, int *out) {
+  static const int pref3[][3] = {{32, 8, 8}, {32, 8, 4}, {16, 8, 8}, {16, 8, 4},
+                                 {16, 4, 4}, {8, 8, 8},  {8, 4, 4},  {4, 4, 4}};
+  static const int pref2[][3] = {{64, 32, 1}, {64, 16, 1}, {32, 16, 1}, {32, 8, 1},
+                                 {16, 8, 1},  {8, 8, 1},   {8, 4, 1},   {4, 4, 1}};
+  const double max_overhead = dim == 3 ? 4.5 : 2.0;
+  int64_t best_units = -1;
+  for (int i = 0; i < 8; ++i) {
+    const int *b = dim == 3 ? pref3[i] : pref2[i];
+    bool fits = true;
+    int64_t units = 1, vol = 1;
+    for (int d = 0; d < dim; ++d) {
+      fits = fits && b[d] <= nf[d];
+      units *= (nf[d] + b[d] - 1) / b[d];
+      vol *= b[d];
+    }
+    const int64_t cells = tile_cells(dim, b, ns);
+    if (!fits || size_t(cells) * cell_bytes > budget) continue;
+    // A tile much larger than its bin (small bins, wide kernel) costs more
+    // staging than the extra workgroups gain.
+    if (double(cells) > max_overhead * double(vol)) continue;
+    if (units >= min_units) {
+      for (int d = 0; d < 3; ++d) out[d] = d < dim ? b[d] : 1;
+      return true;
+    }
+    if (units > best_units) {
+      best_units = units;
+      for (int d = 0; d < 3; ++d) out[d] = d < dim ? b[d] : 1;
+    }
   }
-  return false;
+  return best_units > 0;
 }
 
 }  // namespace
@@ -263,7 +285,6 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
     nftot_ = int64_t(nf_[0]) * nf_[1] * nf_[2];
     nmodes_ = int64_t(ms_[0]) * ms_[1] * ms_[2];
     if (nftot_ > MAX_NF) fail(FINUFFT_ERR_MAXNALLOC, "fine grid exceeds the int32 index range");
-    setup_bins();
   }
   // Vectors per FFT batch: the user's choice, else up to 8 vectors while the
   // batch grid stays within 2 GiB.
@@ -273,7 +294,10 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
     const int64_t per = std::max<int64_t>(nftot_, 1) * int64_t(sizeof(C));
     batchsize_ = int(std::max<int64_t>(1, std::min<int64_t>({ntransf_, 8, (int64_t(2) << 30) / per})));
   }
-  if (type_ != 3) prepare_grid();
+  if (type_ != 3) {
+    setup_bins();  // the bin choice counts the vectors of a batch
+    prepare_grid();
+  }
 }
 
 template <typename T>
@@ -376,10 +400,11 @@ void Plan<T>::setup_bins() {
       for (int d = 0; d < 3; ++d) bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : 8) : 1;
       ok = size_t(tile_cells(dim_, bs, ns_)) * cell <= size_t(lds_limit);
     } else {
-      const int64_t min_units = 1024;
-      ok = !spreads && dim_ == 3 &&
-           choose_interp_bins_3d(nf_, ns_, cell, std::min<size_t>(size_t(lds_limit), 64 * 1024),
-                                 min_units, bs);
+      // about 1024 workgroups per launch (bins x vectors of a batch)
+      const int64_t min_units = (1024 + batchsize_ - 1) / batchsize_;
+      ok = !spreads && dim_ >= 2 &&
+           choose_interp_bins(dim_, nf_, ns_, cell, std::min<size_t>(size_t(lds_limit), 64 * 1024),
+                              min_units, bs);
       if (!ok)
         ok = spreads && choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), true, min_units, bs);
       if (!ok) ok = choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit), false, min_units, bs);

@@ -37,6 +37,7 @@ __global__ void fseries_kernel(int dim, int nf0, int nf1, int nf2, int q, const 
 struct ModeMap {
   int dim, modeord;
   int ms[3], nf[3];
+  double rms[3], rnf[3];  // reciprocals (index arithmetic by multiplication)
 };
 
 // Frequency of mode index k along one axis of length m.
@@ -44,69 +45,120 @@ __device__ __forceinline__ int freq_of(int k, int m, int modeord) {
   return modeord == 0 ? k - m / 2 : (k < m - m / 2 ? k : k - m);
 }
 
+// q = n / d and n - q d for 0 <= n, d < 2^31, with rd = 1 / d in double:
+// the product is within 2^-21 of n / d, so one correction step is exact.
+__device__ __forceinline__ int divmod31(int n, int d, double rd, int &r) {
+  int q = int(double(n) * rd);
+  r = n - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  return q;
+}
+
 template <typename T>
-__global__ void deconv_to_modes_kernel(ModeMap mm, const T *k0, const T *k1, const T *k2,
-                                       const cplx_t<T> *fw, cplx_t<T> *fk, int64_t nmodes,
-                                       int64_t nftot) {
+using vec2_t = T __attribute__((ext_vector_type(2)));
+
+// Fine-grid cell g (< 2^31) -> its mode index kidx and the product kv of the
+// kernel's Fourier coefficients, or false outside the modes. The outer axes
+// are tested first, so that a row outside the modes costs one test.
+template <typename T, int DIM>
+__device__ __forceinline__ bool cell_to_mode(int g, const int *nf, const double *rnf, const int *ms,
+                                             int modeord, const gptr<const T> *ker, int &kidx,
+                                             T &kv) {
+  int c[DIM];
+  int rem = g;
+#pragma unroll
+  for (int d = 0; d < DIM - 1; ++d) rem = divmod31(rem, nf[d], rnf[d], c[d]);
+  c[DIM - 1] = rem;
+  kidx = 0;
+  kv = T(1);
+#pragma unroll
+  for (int d = DIM - 1; d >= 0; --d) {
+    const int m = ms[d], gd = c[d];
+    int fr;
+    if (gd < m - m / 2)
+      fr = gd;
+    else if (gd >= nf[d] - m / 2)
+      fr = gd - nf[d];
+    else
+      return false;
+    const int k = modeord == 0 ? fr + m / 2 : (fr >= 0 ? fr : m + fr);
+    kv *= ker[d][fr >= 0 ? fr : -fr];
+    kidx = kidx * m + k;
+  }
+  return true;
+}
+
+// Mode i (< 2^31) -> its fine-grid cell and the kernel product.
+template <typename T, int DIM>
+__device__ __forceinline__ int mode_to_cell(int i, const ModeMap &mm, const gptr<const T> *ker,
+                                            T &kv) {
+  int k[DIM];
+  int rem = i;
+#pragma unroll
+  for (int d = 0; d < DIM - 1; ++d) rem = divmod31(rem, mm.ms[d], mm.rms[d], k[d]);
+  k[DIM - 1] = rem;
+  int widx = 0;
+  kv = T(1);
+#pragma unroll
+  for (int d = DIM - 1; d >= 0; --d) {
+    const int fr = freq_of(k[d], mm.ms[d], mm.modeord);
+    kv *= ker[d][fr >= 0 ? fr : -fr];
+    widx = widx * mm.nf[d] + (fr >= 0 ? fr : mm.nf[d] + fr);
+  }
+  return widx;
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(EW_T)
+    deconv_to_modes_kernel(ModeMap mm, const T *k0, const T *k1, const T *k2,
+                           const cplx_t<T> *fw, cplx_t<T> *fk, int64_t nmodes, int64_t nftot) {
   const int t = blockIdx.y;
-  const T *ker[3] = {k0, k1, k2};
+  const gptr<const T> ker[3] = {to_global(k0), to_global(k1), to_global(k2)};
+  const gptr<const vec2_t<T>> src =
+      to_global(reinterpret_cast<const vec2_t<T> *>(fw)) + int64_t(t) * nftot;
+  const gptr<vec2_t<T>> dst = to_global(reinterpret_cast<vec2_t<T> *>(fk)) + int64_t(t) * nmodes;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nmodes;
        i += int64_t(gridDim.x) * blockDim.x) {
-    int64_t rem = i, widx = 0, wstride = 1;
-    T kv = T(1);
-    for (int d = 0; d < mm.dim; ++d) {
-      const int k = int(rem % mm.ms[d]);
-      rem /= mm.ms[d];
-      const int fr = freq_of(k, mm.ms[d], mm.modeord);
-      const int w = fr >= 0 ? fr : mm.nf[d] + fr;
-      kv *= ker[d][fr >= 0 ? fr : -fr];
-      widx += w * wstride;
-      wstride *= mm.nf[d];
-    }
-    const cplx_t<T> v = fw[t * nftot + widx];
-    fk[t * nmodes + i] = cplx_t<T>{v.x / kv, v.y / kv};
+    T kv;
+    const int widx = mode_to_cell<T, DIM>(int(i), mm, ker, kv);
+    const vec2_t<T> v = src[widx];
+    vec2_t<T> o;
+    o.x = v.x / kv;
+    o.y = v.y / kv;
+    dst[i] = o;
   }
 }
 
-template <typename T>
-__global__ void amplify_to_grid_kernel(ModeMap mm, const T *k0, const T *k1, const T *k2,
-                                       cplx_t<T> *fw, const cplx_t<T> *fk, int64_t nmodes,
-                                       int64_t nftot) {
+template <typename T, int DIM>
+__global__ void __launch_bounds__(EW_T)
+    amplify_to_grid_kernel(ModeMap mm, const T *k0, const T *k1, const T *k2, cplx_t<T> *fw,
+                           const cplx_t<T> *fk, int64_t nmodes, int64_t nftot) {
   const int t = blockIdx.y;
-  const T *ker[3] = {k0, k1, k2};
+  const gptr<const T> ker[3] = {to_global(k0), to_global(k1), to_global(k2)};
+  const gptr<const vec2_t<T>> src =
+      to_global(reinterpret_cast<const vec2_t<T> *>(fk)) + int64_t(t) * nmodes;
+  const gptr<vec2_t<T>> dst = to_global(reinterpret_cast<vec2_t<T> *>(fw)) + int64_t(t) * nftot;
   for (int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; g < nftot;
        g += int64_t(gridDim.x) * blockDim.x) {
-    int64_t rem = g, kidx = 0, kstride = 1;
-    T kv = T(1);
-    bool inside = true;
-    for (int d = 0; d < mm.dim; ++d) {
-      const int gd = int(rem % mm.nf[d]);
-      rem /= mm.nf[d];
-      const int m = mm.ms[d];
-      int fr;
-      if (gd < m - m / 2)
-        fr = gd;
-      else if (gd >= mm.nf[d] - m / 2)
-        fr = gd - mm.nf[d];
-      else {
-        inside = false;
-        break;
-      }
-      const int k = mm.modeord == 0 ? fr + m / 2 : (fr >= 0 ? fr : m + fr);
-      kv *= ker[d][fr >= 0 ? fr : -fr];
-      kidx += k * kstride;
-      kstride *= m;
+    int kidx;
+    T kv;
+    vec2_t<T> out = {T(0), T(0)};
+    if (cell_to_mode<T, DIM>(int(g), mm.nf, mm.rnf, mm.ms, mm.modeord, ker, kidx, kv)) {
+      const vec2_t<T> v = src[kidx];
+      out.x = v.x / kv;
+      out.y = v.y / kv;
     }
-    cplx_t<T> out{0, 0};
-    if (inside) {
-      const cplx_t<T> v = fk[t * nmodes + kidx];
-      out = cplx_t<T>{v.x / kv, v.y / kv};
-    }
-    fw[t * nftot + g] = out;
+    dst[g] = out;
   }
 }
 
-template <typename T>
+template <typename T, int DIM>
 __global__ void __launch_bounds__(EW_T)
     amplify_batch_kernel(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
                          cplx_t<T> *const *fw_base, int nprob, int64_t fk_vec_offset,
@@ -122,39 +174,29 @@ __global__ void __launch_bounds__(EW_T)
     else
       hi = mid - 1;
   }
-  const AmpItem<T> &it = items[lo];
+  lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: scalar loads of the descriptor
+  const AmpItem<T> it = items[lo];
+  const gptr<const T> ker[3] = {to_global(it.ker[0]), to_global(it.ker[1]),
+                                to_global(it.ker[2])};
   const int64_t local = b - it.blk_begin;
   const int t = int(local / it.chunks);
-  const int64_t g0 = (local % it.chunks) * int64_t(AMP_CHUNK);
-  const cplx_t<T> *fk = fk_base[lo] + fk_vec_offset * it.nmodes + int64_t(t) * it.nmodes;
-  cplx_t<T> *fw = fw_base[lo] + int64_t(t) * it.nftot;
-  const int64_t g1 = g0 + AMP_CHUNK < it.nftot ? g0 + AMP_CHUNK : it.nftot;
-  for (int64_t g = g0 + threadIdx.x; g < g1; g += EW_T) {
-    int64_t rem = g, kidx = 0, kstride = 1;
-    T kv = T(1);
-    bool inside = true;
-    for (int d = 0; d < it.dim; ++d) {
-      const int gd = int(rem % it.nf[d]);
-      rem /= it.nf[d];
-      const int m = it.ms[d];
-      int fr;
-      if (gd < m - m / 2)
-        fr = gd;
-      else if (gd >= it.nf[d] - m / 2)
-        fr = gd - it.nf[d];
-      else {
-        inside = false;
-        break;
-      }
-      const int k = it.modeord == 0 ? fr + m / 2 : (fr >= 0 ? fr : m + fr);
-      kv *= it.ker[d][fr >= 0 ? fr : -fr];
-      kidx += k * kstride;
-      kstride *= m;
-    }
-    cplx_t<T> out{0, 0};
-    if (inside) {
-      const cplx_t<T> v = fk[kidx];
-      out = cplx_t<T>{v.x / kv, v.y / kv};
+  // Grid indices fit 31 bits (the plan caps a vector's grid there).
+  const int g0 = int(local % it.chunks) * AMP_CHUNK;
+  const int g1 = g0 + AMP_CHUNK < it.nftot ? g0 + AMP_CHUNK : int(it.nftot);
+  const gptr<const vec2_t<T>> fk =
+      to_global(reinterpret_cast<const vec2_t<T> *>(fk_base[lo])) +
+      (fk_vec_offset + t) * it.nmodes;
+  const gptr<vec2_t<T>> fw = to_global(reinterpret_cast<vec2_t<T> *>(fw_base[lo])) +
+                             int64_t(t) * it.nftot;
+  for (int o = int(threadIdx.x); o < g1 - g0; o += EW_T) {
+    const int g = g0 + o;
+    int kidx;
+    T kv;
+    vec2_t<T> out = {T(0), T(0)};
+    if (cell_to_mode<T, DIM>(g, it.nf, it.rnf, it.ms, it.modeord, ker, kidx, kv)) {
+      const vec2_t<T> v = fk[kidx];
+      out.x = v.x / kv;
+      out.y = v.y / kv;
     }
     fw[g] = out;
   }
@@ -273,30 +315,37 @@ template <typename T>
 void deconvolve(int dim, int modeord, bool to_modes, const int ms[3], const int nf[3],
                 const T *const ker[3], cplx_t<T> *fw, cplx_t<T> *fk, int blk,
                 hipStream_t stream) {
-  ModeMap mm{dim, modeord, {ms[0], ms[1], ms[2]}, {nf[0], nf[1], nf[2]}};
+  ModeMap mm{dim, modeord, {ms[0], ms[1], ms[2]}, {nf[0], nf[1], nf[2]}, {}, {}};
+  for (int d = 0; d < 3; ++d) {
+    mm.rms[d] = 1.0 / double(ms[d]);
+    mm.rnf[d] = 1.0 / double(nf[d]);
+  }
   const int64_t nmodes = int64_t(ms[0]) * ms[1] * ms[2];
   const int64_t nftot = int64_t(nf[0]) * nf[1] * nf[2];
   const T *k1 = dim > 1 ? ker[1] : ker[0], *k2 = dim > 2 ? ker[2] : ker[0];
   if (to_modes) {
-    deconv_to_modes_kernel<T><<<ew_grid(nmodes, blk), EW_T, 0, stream>>>(mm, ker[0], k1, k2, fw,
-                                                                          fk, nmodes, nftot);
+    auto k = dim == 1 ? deconv_to_modes_kernel<T, 1>
+                      : (dim == 2 ? deconv_to_modes_kernel<T, 2> : deconv_to_modes_kernel<T, 3>);
+    k<<<ew_grid(nmodes, blk), EW_T, 0, stream>>>(mm, ker[0], k1, k2, fw, fk, nmodes, nftot);
     launch_check("deconv_to_modes_kernel");
   } else {
-    amplify_to_grid_kernel<T><<<ew_grid(nftot, blk), EW_T, 0, stream>>>(mm, ker[0], k1, k2, fw,
-                                                                         fk, nmodes, nftot);
+    auto k = dim == 1 ? amplify_to_grid_kernel<T, 1>
+                      : (dim == 2 ? amplify_to_grid_kernel<T, 2> : amplify_to_grid_kernel<T, 3>);
+    k<<<ew_grid(nftot, blk), EW_T, 0, stream>>>(mm, ker[0], k1, k2, fw, fk, nmodes, nftot);
     launch_check("amplify_to_grid_kernel");
   }
 }
 
 template <typename T>
 void amplify_batch(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
-                   cplx_t<T> *const *fw_base, int nprob, int nvec, int64_t fk_vec_offset,
+                   cplx_t<T> *const *fw_base, int nprob, int dim, int64_t fk_vec_offset,
                    int64_t total_blocks, hipStream_t stream) {
-  (void)nvec;
   if (total_blocks <= 0 || nprob <= 0) return;
   if (total_blocks > INT32_MAX) fail(FINUFFT_ERR_MAXNALLOC, "amplify_batch: grid too large");
-  amplify_batch_kernel<T><<<unsigned(total_blocks), EW_T, 0, stream>>>(
-      items, fk_base, fw_base, nprob, fk_vec_offset, total_blocks);
+  auto k = dim == 1 ? amplify_batch_kernel<T, 1>
+                    : (dim == 2 ? amplify_batch_kernel<T, 2> : amplify_batch_kernel<T, 3>);
+  k<<<unsigned(total_blocks), EW_T, 0, stream>>>(items, fk_base, fw_base, nprob, fk_vec_offset,
+                                                 total_blocks);
   launch_check("amplify_batch_kernel");
 }
 

@@ -36,6 +36,7 @@ template <typename T>
 struct AmpItem {
   int dim, modeord;
   int ms[3], nf[3];
+  double rnf[3];  // 1 / nf (index arithmetic by reciprocal multiplication)
   const T *ker[3];
   int64_t nmodes, nftot;
   int64_t blk_begin;
@@ -48,7 +49,7 @@ constexpr int AMP_CHUNK = 4096;
 // of nvec * chunks. Writes every fine-grid cell (zeros outside the modes).
 template <typename T>
 void amplify_batch(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
-                   cplx_t<T> *const *fw_base, int nprob, int nvec, int64_t fk_vec_offset,
+                   cplx_t<T> *const *fw_base, int nprob, int dim, int64_t fk_vec_offset,
                    int64_t total_blocks, hipStream_t stream);
 
 // Sets n complex values to zero (graph-capture friendly, no memset node).

@@ -680,6 +680,108 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
   }
 }
 
+// Interpolation of one subproblem over nvec vectors (the fused launch over
+// many problems). The points are taken in chunks of up to two per thread;
+// each point's kernel weights and tile offsets stay in registers while the
+// tile of every vector in turn is staged and applied, so a point's kernel is
+// evaluated once per round rather than once per vector. Global arrays are
+// accessed through global-typed pointers and the Horner table through a
+// constant-typed one (scalar loads).
+template <typename T, int DIM, int NS>
+__device__ __forceinline__ void interp_unit_mv(const SpreadCtx<T> &ctx, const KerParams<T> &kp,
+                                               const StartTile<DIM, NS> &st, cplx_t<T> *tile,
+                                               gptr<const T> g0, int64_t nftot, gptr<T> c0,
+                                               int nvec) {
+  constexpr int PT = 2;  // points per thread per chunk
+  constexpr int CH = PT * BIN_THREADS;
+  const gptr<const int> idx = to_global(ctx.idx);
+  const kptr<T> coef = to_const(kp.coef);
+  gptr<const T> sx[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sx[d] = to_global(ctx.sxyz[d]);
+  for (int c0i = 0; c0i < st.npts; c0i += CH) {
+    const int n = min(CH, st.npts - c0i);
+    // Strided dealing (see interp_unit): item i = 32 m + r takes point r G + m.
+    const int G = (n + 31) >> 5;
+    T ker[PT][DIM][NS];
+    int l[PT][DIM];
+    int j[PT];
+    bool on[PT];
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int i = threadIdx.x + k * BIN_THREADS;
+      const int pos = (i & 31) * G + (i >> 5);
+      on[k] = i < 32 * G && pos < n;
+      j[k] = 0;
+      if (on[k]) {
+        const int q = st.first + c0i + pos;
+        j[k] = idx[q];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+          l[k][d] = st.local_start(
+              start_and_kernel_c<T, NS>(sx[d][q], ctx.nf[d], ker[k][d], kp, coef), d, ctx.nf[d]);
+      }
+    }
+    for (int v = 0; v < nvec; ++v) {
+      __syncthreads();  // the previous tile is no longer read
+      const gptr<const T> g = g0 + 2 * int64_t(v) * nftot;
+      for (int m = threadIdx.x; m < st.ntile; m += BIN_THREADS) {
+        const int64_t gi = 2 * st.grid_index(ctx, m);
+        tile[m] = cplx_t<T>{g[gi], g[gi + 1]};
+      }
+      __syncthreads();
+      const gptr<T> cout = c0 + 2 * int64_t(v) * ctx.M;
+#pragma unroll
+      for (int k = 0; k < PT; ++k) {
+        if (!on[k]) continue;
+        T ar = 0, ai = 0;
+        if constexpr (DIM == 1) {
+#pragma unroll
+          for (int a = 0; a < NS; ++a) {
+            const cplx_t<T> w = tile[l[k][0] + a];
+            ar = std::fma(w.x, ker[k][0][a], ar);
+            ai = std::fma(w.y, ker[k][0][a], ai);
+          }
+        } else if constexpr (DIM == 2) {
+          for (int b = 0; b < NS; ++b) {
+            const cplx_t<T> *row = tile + (l[k][1] + b) * st.P[0] + l[k][0];
+            T rr = 0, ri = 0;
+#pragma unroll
+            for (int a = 0; a < NS; ++a) {
+              const cplx_t<T> w = row[a];
+              rr = std::fma(w.x, ker[k][0][a], rr);
+              ri = std::fma(w.y, ker[k][0][a], ri);
+            }
+            ar = std::fma(rr, ker[k][1][b], ar);
+            ai = std::fma(ri, ker[k][1][b], ai);
+          }
+        } else {
+          for (int e = 0; e < NS; ++e) {
+            T pr = 0, pi = 0;
+            for (int b = 0; b < NS; ++b) {
+              const cplx_t<T> *row =
+                  tile + ((l[k][2] + e) * st.P[1] + l[k][1] + b) * st.P[0] + l[k][0];
+              T rr = 0, ri = 0;
+#pragma unroll
+              for (int a = 0; a < NS; ++a) {
+                const cplx_t<T> w = row[a];
+                rr = std::fma(w.x, ker[k][0][a], rr);
+                ri = std::fma(w.y, ker[k][0][a], ri);
+              }
+              pr = std::fma(rr, ker[k][1][b], pr);
+              pi = std::fma(ri, ker[k][1][b], pi);
+            }
+            ar = std::fma(pr, ker[k][2][e], ar);
+            ai = std::fma(pi, ker[k][2][e], ai);
+          }
+        }
+        cout[2 * int64_t(j[k])] = ar;
+        cout[2 * int64_t(j[k]) + 1] = ai;
+      }
+    }
+  }
+}
+
 // Bin interpolation: stage the padded tile, then gather per point. The unit is
 // a whole bin, or (interp_sub) a subproblem of at most maxsub points of one
 // bin, so that a dense bin is shared by several workgroups.
@@ -772,37 +874,37 @@ __global__ void __launch_bounds__(BIN_THREADS)
   interp_unit<T, DIM, NS>(ctx, st, tile, fw + int64_t(t) * nftot, c + int64_t(t) * ctx.M);
 }
 
-// Persistent interpolation of many problems (see interp_batch): each
-// workgroup walks work units blockIdx.x, blockIdx.x + gridDim.x, ...
+// Interpolation of many problems (see interp_batch): workgroup b takes work
+// unit b (a subproblem of one problem, over all vectors of the round).
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(BIN_THREADS)
-    interp_batch_kernel(const InterpItem<T> *__restrict__ items, int nprob,
+    interp_batch_kernel(const InterpItem<T> *__restrict__ items, int nprob, KerParams<T> kp,
                         const cplx_t<T> *const *__restrict__ fw_base,
-                        cplx_t<T> *const *__restrict__ c_base, int64_t c_vec_offset,
+                        cplx_t<T> *const *__restrict__ c_base, int64_t c_vec_offset, int nvec,
                         int64_t total_units) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cplx_t<T> *tile = reinterpret_cast<cplx_t<T> *>(lds_raw);
-  for (int64_t u = blockIdx.x; u < total_units; u += gridDim.x) {
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (items[mid].unit_begin <= u)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const InterpItem<T> &it = items[lo];
-    const int64_t local = u - it.unit_begin;
-    const int t = int(local / it.nsub);
-    const int s = int(local % it.nsub);
-    StartTile<DIM, NS> st;
-    const bool ok = st.init_sub(it.ctx, s);
-    if (ok && st.npts > 0) {
-      interp_unit<T, DIM, NS>(it.ctx, st, tile, fw_base[lo] + int64_t(t) * it.nftot,
-                              c_base[lo] + (c_vec_offset + t) * int64_t(it.ctx.M));
-    }
-    __syncthreads();  // the tile is reused by the next unit
+  const int64_t u = blockIdx.x;
+  if (u >= total_units) return;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].unit_begin <= u)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
+  // lo is wave-uniform (readfirstlane: scalar loads of the descriptor);
+  // interp_unit_mv accesses the problem's arrays through global-typed
+  // pointers, as a kernel with the context as argument does (generic pointers
+  // cost flat loads and ~100 vector registers, halving the occupancy).
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const InterpItem<T> it = items[lo];
+  StartTile<DIM, NS> st;
+  if (!st.init_sub(it.ctx, int(u - it.unit_begin)) || st.npts == 0) return;
+  interp_unit_mv<T, DIM, NS>(
+      it.ctx, kp, st, tile, to_global(reinterpret_cast<const T *>(fw_base[lo])), it.nftot,
+      to_global(reinterpret_cast<T *>(c_base[lo])) + 2 * c_vec_offset * int64_t(it.ctx.M), nvec);
 }
 
 // Interpolant and coordinate derivatives, one lane per point (sorted order),
@@ -994,16 +1096,16 @@ void interp_ns(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int b
 }
 
 template <typename T, int DIM, int NS>
-void interp_batch_ns(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *fw_base,
-                     cplx_t<T> *const *c_base, int64_t c_vec_offset, int64_t total_units,
-                     size_t lds_bytes, hipStream_t stream) {
+void interp_batch_ns(const InterpItem<T> *items, int nprob, const KerParams<T> &kp,
+                     const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
+                     int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
+                     hipStream_t stream) {
   auto k = interp_batch_kernel<T, DIM, NS>;
   allow_lds(k, lds_bytes);
   if (total_units <= 0) return;
-  // persistent grid: enough workgroups to fill the GPU several times over
-  const int64_t g = std::min<int64_t>(total_units, 256 * 16);
-  k<<<unsigned(g), BIN_THREADS, lds_bytes, stream>>>(items, nprob, fw_base, c_base, c_vec_offset,
-                                                     total_units);
+  if (total_units > INT32_MAX) fail(FINUFFT_ERR_MAXNALLOC, "interp_batch: too many work units");
+  k<<<unsigned(total_units), BIN_THREADS, lds_bytes, stream>>>(items, nprob, kp, fw_base, c_base,
+                                                               c_vec_offset, nvec, total_units);
   launch_check("interp_batch_kernel");
 }
 
@@ -1049,11 +1151,12 @@ struct InterpOp {
 
 template <typename T, int DIM, int NS>
 struct InterpBatchOp {
-  static void go(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *fw_base,
-                 cplx_t<T> *const *c_base, int64_t c_vec_offset, int64_t total_units,
-                 size_t lds_bytes, hipStream_t s) {
-    interp_batch_ns<T, DIM, NS>(items, nprob, fw_base, c_base, c_vec_offset, total_units,
-                                lds_bytes, s);
+  static void go(const InterpItem<T> *items, int nprob, const KerParams<T> &kp,
+                 const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
+                 int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
+                 hipStream_t s) {
+    interp_batch_ns<T, DIM, NS>(items, nprob, kp, fw_base, c_base, c_vec_offset, nvec,
+                                total_units, lds_bytes, s);
   }
 };
 
@@ -1074,11 +1177,11 @@ void interp_grad_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const
 }
 
 template <typename T, int DIM>
-void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns,
+void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns, const KerParams<T> &kp,
                       const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
-                      int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                      int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
                       hipStream_t stream) {
-  ns_dispatch<T, InterpBatchOp>::run(ns, items, nprob, fw_base, c_base, c_vec_offset,
+  ns_dispatch<T, InterpBatchOp>::run(ns, items, nprob, kp, fw_base, c_base, c_vec_offset, nvec,
                                      total_units, lds_bytes, stream);
 }
 
@@ -1099,9 +1202,10 @@ template void spread_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, const cplx
 template void interp_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, cplx_t<FA_REAL> *,
                                           const cplx_t<FA_REAL> *, int, hipStream_t);
 template void interp_batch_dim<FA_REAL, FA_DIM>(const InterpItem<FA_REAL> *, int, int,
+                                                const KerParams<FA_REAL> &,
                                                 const cplx_t<FA_REAL> *const *,
-                                                cplx_t<FA_REAL> *const *, int64_t, int64_t,
-                                                size_t, hipStream_t);
+                                                cplx_t<FA_REAL> *const *, int64_t, int,
+                                                int64_t, size_t, hipStream_t);
 template void interp_grad_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, cplx_t<FA_REAL> *,
                                                cplx_t<FA_REAL> *, const cplx_t<FA_REAL> *, int,
                                                hipStream_t);

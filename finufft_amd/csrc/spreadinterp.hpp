@@ -75,6 +75,25 @@ struct SpreadCtx {
   int n_dense_sub = 0;
 };
 
+// Global-memory pointer types. A pointer read from memory (a work
+// descriptor) is generic to the compiler, which then uses flat instructions
+// and cannot load through it with scalar loads; holding it in one of these
+// types keeps every access a global (or, where uniform, scalar) one.
+template <typename P>
+using gptr = __attribute__((address_space(1))) P *;
+template <typename P>
+__device__ __forceinline__ gptr<P> to_global(P *p) {
+  return (gptr<P>)p;
+}
+// Read-only tables (the Horner coefficients): the constant address space,
+// whose loads the compiler may always make scalar.
+template <typename P>
+using kptr = __attribute__((address_space(4))) const P *;
+template <typename P>
+__device__ __forceinline__ kptr<P> to_const(const P *p) {
+  return (kptr<P>)p;
+}
+
 // Column-window spreading geometry. A wave owns the footprint of one
 // sub-column: 8 x 8 (x, y) grid columns, one per lane, for the points whose
 // first node lies in an S x S square, S the largest power of two with
@@ -111,18 +130,21 @@ template <typename T>
 struct InterpItem {
   SpreadCtx<T> ctx;   // interp_sub = 1: substart / sub2bin cover every bin
   int64_t nftot;      // fine-grid cells per vector
-  int nsub;           // subproblems of the problem (work units per vector)
+  int nsub;           // subproblems of the problem (its work units)
   int64_t unit_begin;  // first work unit of this problem in the launch
 };
 
-// Interpolates all problems of a round in one persistent launch: work unit u
-// of problem p is (vector u / nsub, subproblem u % nsub); fw_base[p] is the
-// problem's vector-0 grid of the round, c_base[p] its value array (vector
-// c_vec_offset is the round's first). All problems share the kernel width ns.
+// Interpolates all problems of a launch, one workgroup per work unit: unit u
+// of problem p is its subproblem u - unit_begin, over all nvec vectors of the
+// round (the kernel weights of a point are computed once and applied to
+// every vector). fw_base[p] is the problem's vector-0 grid of the round
+// (vectors nftot apart), c_base[p] its value array (vector c_vec_offset is
+// the round's first). All problems share the kernel (width ns and
+// parameters kp, passed once as launch arguments).
 template <typename T>
-void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns,
+void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns, const KerParams<T> &kp,
                   const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
-                  int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                  int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
                   hipStream_t stream);
 
 // Per-dimension entry points (one translation unit each).
@@ -133,9 +155,9 @@ template <typename T, int DIM>
 void interp_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
                 hipStream_t stream);
 template <typename T, int DIM>
-void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns,
+void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns, const KerParams<T> &kp,
                       const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
-                      int64_t c_vec_offset, int64_t total_units, size_t lds_bytes,
+                      int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
                       hipStream_t stream);
 template <typename T, int DIM>
 void interp_grad_dim(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,

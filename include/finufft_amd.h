@@ -38,9 +38,11 @@ FINUFFT_EXPORT int cufinufftf_interp_grad(cufinufftf_plan d_plan, hipFloatComple
 /* Fused execution of many independent type-2 problems. No reference
  * counterpart: the reference runs one plan per problem. plans[nplans] are
  * type-2 plans (cufinufft_plan or cufinufftf_plan as is_double says) with
- * points set, sharing dimension, ntransf, isign and kernel width, made with
- * gpu_method 0 or 2. arena_bytes bounds the fine-grid memory the batch
- * allocates (0: 16 GiB). The plans must outlive the batch and keep their
+ * points set, sharing dimension, ntransf, isign, tolerance and upsampfac,
+ * made with gpu_method 0 or 2. Problems of equal fine-grid shape are run in
+ * chunks whose grids fit arena_bytes (0: 96 MiB, a size meant to stay in the
+ * GPU's last-level cache from stage to stage); the batch allocates one
+ * chunk's grids. The plans must outlive the batch and keep their
  * points. execute takes host arrays of nplans device pointers: c[p] receives
  * the ntransf x M_p values of problem p, fk[p] holds its ntransf x N_p modes.
  * The whole schedule is recorded into a HIP graph on the first execute and
@@ -51,11 +53,11 @@ FINUFFT_EXPORT int finufft_amd_batch_makeplan(int is_double, int nplans, void *c
 FINUFFT_EXPORT int finufft_amd_batch_execute(finufft_amd_batch batch, void *const *c,
                                              void *const *fk);
 FINUFFT_EXPORT int finufft_amd_batch_destroy(finufft_amd_batch batch);
-/* iout[6]: number of problems, vectors per round, distinct fine-grid shapes,
- * 1 when the schedule runs as a recorded graph, and when recording failed the
+/* iout[7]: number of problems, vectors per round, distinct fine-grid shapes,
+ * 1 when the schedule runs as a recorded graph, when recording failed the
  * stage (1 begin, 2 a call inside the capture, 3 end, 4 instantiate) and its
- * HIP error code. use_graph = 0 runs direct launches instead (for
- * comparison). */
+ * HIP error code, and the number of chunks. use_graph = 0 runs direct
+ * launches instead (for comparison). */
 FINUFFT_EXPORT int finufft_amd_batch_info(finufft_amd_batch batch, int64_t *iout);
 FINUFFT_EXPORT int finufft_amd_batch_set_graph(finufft_amd_batch batch, int use_graph);
 

@@ -108,3 +108,34 @@ def test_batch_rejects_mixed_plans(torch_gpu):
         BatchPlan([p2, p1])
     with pytest.raises(FinufftError):
         BatchPlan([p2, p3])
+
+
+def test_batch_chunks_with_remainder(torch_gpu):
+    """Ten problems of one grid shape, a budget of three per chunk: chunks of
+    3, 3, 3 and 1 reuse one grid arena in turn."""
+    from finufft_amd import Plan
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    r = rng(93)
+    ntr, shape = 2, (16, 16)
+    plans, pts, fks = [], [], []
+    for _ in range(10):
+        M = int(r.integers(50, 900))
+        x = [r.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(2)]
+        p = Plan(2, shape, n_trans=ntr, eps=1e-5, dtype="complex64")
+        p.setpts(*[to_dev(torch, a) for a in x])
+        plans.append(p)
+        pts.append(x)
+        fks.append(to_dev(torch, cnormal(r, (ntr,) + shape, np.dtype("complex64"))))
+    nftot = int(np.prod(plans[0].info()["nf"][:2]))
+    b = BatchPlan(plans, arena_bytes=3 * ntr * nftot * 8)
+    info = b.info()
+    assert info["grid_shapes"] == 1 and info["vectors_per_round"] == ntr
+    assert info["chunks"] == 4, info
+    outs = [torch.empty((ntr, len(x[0])), dtype=torch.complex64, device="cuda") for x in pts]
+    b.execute(fks, outs)
+    torch.cuda.synchronize()
+    for p, fk, o in zip(plans, fks, outs):
+        ref = to_host(p.execute(fk))
+        assert nudft.relerr(to_host(o), ref) < 5e-6
