@@ -2,6 +2,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <string>
 
@@ -72,6 +73,14 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
     }
   }
 
+  // 2-D grids whose sides fit one LDS buffer are made by the fused column /
+  // row transforms; otherwise by amplification and rocFFT.
+  fused_ = dim_ == 2;
+  for (size_t gi = 0; gi < groups_.size() && fused_; ++gi) {
+    Group &g = *groups_[gi];
+    fused_ = fused_fft_shape<T>(g.nf, g.cw, g.lcw, g.rw, g.lds_col, g.lds_row);
+  }
+
   // Chunks: consecutive problems of one group, as many as fit the budget.
   const bool fp64 = std::is_same<T, double>::value;
   int64_t arena_cells = 0;
@@ -83,11 +92,32 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
                                                               int64_t(chunk_bytes) / per)));
     const int K = int(g.probs.size());
     const int rest = K % g.per_chunk;
-    g.fft_full.setup(dim_, g.nf, V_ * g.per_chunk, fp64);
-    g.fft_full.prepare(iflag_);
-    if (rest) {
-      g.fft_rest.setup(dim_, g.nf, V_ * rest, fp64);
-      g.fft_rest.prepare(iflag_);
+    if (fused_) {
+      const Plan<T> &p = *plans_[g.probs[0]];  // the group shares nf and kernel
+      for (int d = 0; d < 2; ++d) {
+        std::vector<T> tw;
+        fft_twiddles<T>(g.nf[d], tw);
+        g.tw[d].resize(tw.size());
+        hip_check(hipMemcpy(g.tw[d].get(), tw.data(), tw.size() * sizeof(T),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy(batch twiddles)");
+        std::vector<T> k(size_t(g.nf[d] / 2 + 1));
+        hip_check(hipMemcpy(k.data(), p.fwker_[d].get(), k.size() * sizeof(T),
+                            hipMemcpyDeviceToHost),
+                  "hipMemcpy(batch kernel series)");
+        for (T &v : k) v = T(1.0 / double(v));
+        g.rker[d].resize(k.size());
+        hip_check(hipMemcpy(g.rker[d].get(), k.data(), k.size() * sizeof(T),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy(batch kernel series)");
+      }
+    } else {
+      g.fft_full.setup(dim_, g.nf, V_ * g.per_chunk, fp64);
+      g.fft_full.prepare(iflag_);
+      if (rest) {
+        g.fft_rest.setup(dim_, g.nf, V_ * rest, fp64);
+        g.fft_rest.prepare(iflag_);
+      }
     }
     for (int k = 0; k < K; k += g.per_chunk) {
       Chunk c;
@@ -95,7 +125,7 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
       c.count = std::min(g.per_chunk, K - k);
       c.group = gi;
       c.full = c.count == g.per_chunk;
-      c.amp_blocks = c.units = 0;
+      c.amp_blocks = c.units = c.col_blocks = c.row_blocks = 0;
       chunks_.push_back(c);
       arena_cells = std::max<int64_t>(arena_cells, int64_t(c.count) * V_ * nft);
     }
@@ -106,9 +136,10 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
   // Work tables in slot order; prefix counts restart at every chunk.
   std::vector<AmpItem<T>> amp(n);
   std::vector<InterpItem<T>> itp(n);
+  std::vector<FusedFftItem<T>> ffi(fused_ ? n : 0);
   std::vector<C *> fwb(n);
   for (Chunk &c : chunks_) {
-    int64_t blocks = 0, units = 0;
+    int64_t blocks = 0, units = 0, cblocks = 0, rblocks = 0;
     for (int s = c.first; s < c.first + c.count; ++s) {
       const int i = slot_prob_[s];
       const Plan<T> &p = *plans_[i];
@@ -136,10 +167,43 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
       units += nsub[i];
       lds_bytes_ = std::max(lds_bytes_, p.lds_bytes_);
 
+      if (fused_) {
+        FusedFftItem<T> &f = ffi[s];
+        const Group &g = *groups_[c.group];
+        f.modeord = p.opts_.modeord;
+        for (int d = 0; d < 2; ++d) {
+          f.ms[d] = p.ms_[d];
+          f.nf[d] = p.nf_[d];
+          f.rker[d] = g.rker[d].get();
+          f.tw[d] = g.tw[d].get();
+          radix_plan(p.nf_[d], f.rp[d]);
+        }
+        f.cw = g.cw;
+        f.lcw = g.lcw;
+        f.rw = g.rw;
+        f.ncol = (f.ms[0] + f.cw - 1) / f.cw;
+        f.nrow = (f.nf[1] + f.rw - 1) / f.rw;
+        f.nmodes = p.nmodes_;
+        f.nftot = p.nftot_;
+        f.col_begin = cblocks;
+        f.row_begin = rblocks;
+        cblocks += int64_t(f.ncol) * V_;
+        rblocks += int64_t(f.nrow) * V_;
+      }
+
       fwb[s] = arena_.get() + int64_t(s - c.first) * V_ * p.nftot_;
     }
     c.amp_blocks = blocks;
     c.units = units;
+    c.col_blocks = cblocks;
+    c.row_blocks = rblocks;
+  }
+  if (fused_) {
+    fused_fft_prepare<T>();
+    fft_items_.resize(n);
+    hip_check(hipMemcpy(fft_items_.get(), ffi.data(), n * sizeof(FusedFftItem<T>),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy(batch transform items)");
   }
   amp_items_.resize(n);
   interp_items_.resize(n);
@@ -178,9 +242,17 @@ void BatchPlan<T>::enqueue(hipStream_t s) {
   for (int v0 = 0; v0 < ntr_; v0 += V_) {
     for (const Chunk &c : chunks_) {
       Group &g = *groups_[c.group];
-      amplify_batch<T>(amp_items_.get() + c.first, fk_base_.get() + c.first,
-                       fw_base_.get() + c.first, c.count, dim_, v0, c.amp_blocks, s);
-      (c.full ? g.fft_full : g.fft_rest).exec(arena_.get(), iflag_, s);
+      if (fused_) {
+        fused_fft_cols<T>(fft_items_.get() + c.first, fk_base_.get() + c.first,
+                          fw_base_.get() + c.first, c.count, v0, iflag_, c.col_blocks, g.lds_col,
+                          s);
+        fused_fft_rows<T>(fft_items_.get() + c.first, fw_base_.get() + c.first, c.count, iflag_,
+                          c.row_blocks, g.lds_row, s);
+      } else {
+        amplify_batch<T>(amp_items_.get() + c.first, fk_base_.get() + c.first,
+                         fw_base_.get() + c.first, c.count, dim_, v0, c.amp_blocks, s);
+        (c.full ? g.fft_full : g.fft_rest).exec(arena_.get(), iflag_, s);
+      }
       interp_batch<T>(interp_items_.get() + c.first, c.count, dim_, ns_, kp_,
                       fw_base_.get() + c.first, c_base_.get() + c.first, v0, V_, c.units,
                       lds_bytes_, s);

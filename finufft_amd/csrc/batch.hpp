@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "fft.hpp"
+#include "fused_fft.hpp"
 #include "plan.hpp"
 #include "plan_kernels.hpp"
 
@@ -47,6 +48,9 @@ class BatchPlan {
   int vectors_per_round() const { return V_; }
   int groups() const { return int(groups_.size()); }
   int chunks() const { return int(chunks_.size()); }
+  // 1 when the grids are made by the fused column/row transforms (2-D grids
+  // whose sides fit one LDS buffer), 0 when by amplification + rocFFT.
+  bool fused_fft() const { return fused_; }
   bool graph() const { return gexec_ != nullptr; }
   // Why the schedule could not be recorded (0: it was, or not tried yet):
   // capture stage (1 begin, 2 a call inside, 3 end, 4 instantiate) and the
@@ -60,13 +64,16 @@ class BatchPlan {
     int nf[3];
     std::vector<int> probs;
     int per_chunk = 1;  // problems per full chunk
-    FftPlan fft_full, fft_rest;
+    FftPlan fft_full, fft_rest;  // unused on the fused 2-D path
+    dbuf<T> tw[2], rker[2];      // twiddles and reciprocal kernel series (fused 2-D path)
+    int cw = 1, lcw = 0, rw = 1;  // fused 2-D path: columns / rows per workgroup
+    size_t lds_col = 0, lds_row = 0;
   };
   struct Chunk {
     int first, count;  // slots [first, first + count)
     int group;
     bool full;
-    int64_t amp_blocks, units;
+    int64_t amp_blocks, units, col_blocks, row_blocks;
   };
   void enqueue(hipStream_t s);
 
@@ -83,7 +90,9 @@ class BatchPlan {
   std::vector<std::unique_ptr<Group>> groups_;
   std::vector<Chunk> chunks_;
   std::vector<int> slot_prob_;  // problem index of each slot (group-major order)
+  bool fused_ = false;
   dbuf<AmpItem<T>> amp_items_;
+  dbuf<FusedFftItem<T>> fft_items_;
   dbuf<InterpItem<T>> interp_items_;
   dbuf<C *> fw_base_;
   dbuf<const C *> fk_base_;

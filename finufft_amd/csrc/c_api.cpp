@@ -120,7 +120,7 @@ std::unique_ptr<fa::BatchPlan<T>> make_batch(int n, void *const *plans, size_t a
   if (n < 1 || plans == nullptr) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: no plans");
   std::vector<Plan<T> *> v(n);
   for (int i = 0; i < n; ++i) v[i] = as_plan<T>(plans[i]);
-  const size_t budget = arena ? arena : (size_t(96) << 20);
+  const size_t budget = arena ? arena : (size_t(192) << 20);
   const hipStream_t stream = v[0]->stream();
   return std::make_unique<fa::BatchPlan<T>>(std::move(v), budget, stream);
 }

@@ -180,6 +180,18 @@ __global__ void __launch_bounds__(SPREAD_THREADS)
 
 constexpr int BIN_THREADS = 512;
 
+// Grid coordinate of a tile coordinate g (tile offset + position). A tile
+// extends at most binsize + ns - 2 cells past the start of its bin, bins start
+// below nf, and binsize <= nf, ns < nf: g < 3 nf, so two conditional
+// subtractions bring it into [0, nf).
+__device__ __forceinline__ int wrap_grid(int g, int nf) {
+  g = g >= nf ? g - nf : g;
+  return g >= nf ? g - nf : g;
+}
+
+template <typename T>
+using vec2_t = T __attribute__((ext_vector_type(2)));
+
 // One LDS work unit of method 2: a bin (all of its points) or a subproblem
 // (<= maxsub points of one bin). Bins are bins of the first support node
 // (the sort's start_bins mode), so a bin of b nodes owns exactly the padded
@@ -242,8 +254,7 @@ struct StartTile {
     for (int d = 0; d < DIM; ++d) {
       const int nd = n % P[d];
       n /= P[d];
-      int g = nd + off[d];
-      g = g >= ctx.nf[d] ? g - ctx.nf[d] : g;
+      const int g = wrap_grid(nd + off[d], ctx.nf[d]);
       gi += g * stride;
       stride *= ctx.nf[d];
     }
@@ -489,10 +500,9 @@ __device__ __forceinline__ int64_t col_grid_index(const SpreadCtx<float> &ctx,
   constexpr int P = col_tile_xy<NS>();
   const int x = n % P, t = n / P;
   const int y = t % P, z = t / P;
-  int gx = x + st.off[0], gy = y + st.off[1], gz = z + st.off[2];
-  gx = gx >= ctx.nf[0] ? gx - ctx.nf[0] : gx;
-  gy = gy >= ctx.nf[1] ? gy - ctx.nf[1] : gy;
-  gz = gz >= ctx.nf[2] ? gz - ctx.nf[2] : gz;
+  const int gx = wrap_grid(x + st.off[0], ctx.nf[0]);
+  const int gy = wrap_grid(y + st.off[1], ctx.nf[1]);
+  const int gz = wrap_grid(z + st.off[2], ctx.nf[2]);
   return (int64_t(gz) * ctx.nf[1] + gy) * ctx.nf[0] + gx;
 }
 
@@ -687,29 +697,102 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
 // evaluated once per round rather than once per vector. Global arrays are
 // accessed through global-typed pointers and the Horner table through a
 // constant-typed one (scalar loads).
+#ifndef FA_MV_THREADS
+#define FA_MV_THREADS 512
+#endif
+#ifndef FA_MV_PT
+#define FA_MV_PT 2
+#endif
+#ifndef FA_MV_PF
+#define FA_MV_PF 6
+#endif
+// Workgroup size of the multi-vector interpolation, points per thread per
+// chunk, and tile cells per thread held in registers while the previous
+// vector is interpolated (tiles larger than MV_PF * MV_THREADS cells are
+// staged without the overlap).
+constexpr int MV_THREADS = FA_MV_THREADS;
+constexpr int MV_PT = FA_MV_PT;
+constexpr int MV_PF = FA_MV_PF;
+
+// Walks a tile in steps of MV_THREADS cells: each thread's first cell and
+// the step in tile coordinates are found once per unit, so that the
+// per-vector loads need no division.
+template <int DIM>
+struct TileWalk {
+  int x0, y0, z0, sx, sy, sz;
+  __device__ void init(const int *P) {
+    int r = threadIdx.x;
+    x0 = r % P[0];
+    r /= P[0];
+    y0 = DIM > 1 ? r % P[1] : 0;
+    z0 = DIM > 1 ? r / P[1] : 0;
+    r = MV_THREADS;
+    sx = r % P[0];
+    r /= P[0];
+    sy = DIM > 1 ? r % P[1] : 0;
+    sz = DIM > 1 ? r / P[1] : 0;
+  }
+  // Grid index of tile cell (x, y, z) of a tile at off (< 2^31: the plan
+  // caps a vector's grid there), then the next cell of the walk.
+  template <typename T>
+  __device__ __forceinline__ int index_and_step(const SpreadCtx<T> &ctx, const int *P,
+                                                const int *off, int &x, int &y, int &z) const {
+    int gi = wrap_grid(x + off[0], ctx.nf[0]);
+    if (DIM > 1) {
+      int row = wrap_grid(y + off[1], ctx.nf[1]);
+      if (DIM > 2) row += wrap_grid(z + off[2], ctx.nf[2]) * ctx.nf[1];
+      gi += row * ctx.nf[0];
+    }
+    x += sx;
+    if (x >= P[0]) {
+      x -= P[0];
+      ++y;
+    }
+    if (DIM > 1) {
+      y += sy;
+      if (y >= P[1]) {
+        y -= P[1];
+        ++z;
+      }
+      z += sz;
+    }
+    return gi;
+  }
+};
+
+// Interpolation of one subproblem over nvec vectors (the fused launch over
+// many problems). The points are taken in chunks of up to MV_PT per thread;
+// each point's kernel weights and tile offsets stay in registers while the
+// tile of every vector in turn is staged and applied, so a point's kernel is
+// evaluated once per round rather than once per vector. The next vector's
+// tile is loaded into registers while the current one is interpolated.
+// Global arrays are accessed through global-typed pointers and the Horner
+// table through a constant-typed one (scalar loads).
 template <typename T, int DIM, int NS>
 __device__ __forceinline__ void interp_unit_mv(const SpreadCtx<T> &ctx, const KerParams<T> &kp,
                                                const StartTile<DIM, NS> &st, cplx_t<T> *tile,
-                                               gptr<const T> g0, int64_t nftot, gptr<T> c0,
-                                               int nvec) {
-  constexpr int PT = 2;  // points per thread per chunk
-  constexpr int CH = PT * BIN_THREADS;
+                                               gptr<const vec2_t<T>> g0, int64_t nftot,
+                                               gptr<vec2_t<T>> c0, int nvec) {
+  constexpr int CH = MV_PT * MV_THREADS;
   const gptr<const int> idx = to_global(ctx.idx);
   const kptr<T> coef = to_const(kp.coef);
   gptr<const T> sx[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) sx[d] = to_global(ctx.sxyz[d]);
+  TileWalk<DIM> walk;
+  walk.init(st.P);
+  const bool pipelined = st.ntile <= MV_PF * MV_THREADS;
   for (int c0i = 0; c0i < st.npts; c0i += CH) {
     const int n = min(CH, st.npts - c0i);
     // Strided dealing (see interp_unit): item i = 32 m + r takes point r G + m.
     const int G = (n + 31) >> 5;
-    T ker[PT][DIM][NS];
-    int l[PT][DIM];
-    int j[PT];
-    bool on[PT];
+    T ker[MV_PT][DIM][NS];
+    int l[MV_PT][DIM];
+    int j[MV_PT];
+    bool on[MV_PT];
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-      const int i = threadIdx.x + k * BIN_THREADS;
+    for (int k = 0; k < MV_PT; ++k) {
+      const int i = threadIdx.x + k * MV_THREADS;
       const int pos = (i & 31) * G + (i >> 5);
       on[k] = i < 32 * G && pos < n;
       j[k] = 0;
@@ -722,17 +805,44 @@ __device__ __forceinline__ void interp_unit_mv(const SpreadCtx<T> &ctx, const Ke
               start_and_kernel_c<T, NS>(sx[d][q], ctx.nf[d], ker[k][d], kp, coef), d, ctx.nf[d]);
       }
     }
+    vec2_t<T> pre[MV_PF];
+    if (pipelined) {  // vector 0's tile into registers
+      int x = walk.x0, y = walk.y0, z = walk.z0;
+#pragma unroll
+      for (int k = 0; k < MV_PF; ++k) {
+        const int m = threadIdx.x + k * MV_THREADS;
+        if (m < st.ntile) pre[k] = g0[walk.index_and_step(ctx, st.P, st.off, x, y, z)];
+      }
+    }
     for (int v = 0; v < nvec; ++v) {
       __syncthreads();  // the previous tile is no longer read
-      const gptr<const T> g = g0 + 2 * int64_t(v) * nftot;
-      for (int m = threadIdx.x; m < st.ntile; m += BIN_THREADS) {
-        const int64_t gi = 2 * st.grid_index(ctx, m);
-        tile[m] = cplx_t<T>{g[gi], g[gi + 1]};
+      if (pipelined) {
+#pragma unroll
+        for (int k = 0; k < MV_PF; ++k) {
+          const int m = threadIdx.x + k * MV_THREADS;
+          if (m < st.ntile) tile[m] = cplx_t<T>{pre[k].x, pre[k].y};
+        }
+      } else {
+        const gptr<const vec2_t<T>> g = g0 + int64_t(v) * nftot;
+        int x = walk.x0, y = walk.y0, z = walk.z0;
+        for (int m = threadIdx.x; m < st.ntile; m += MV_THREADS) {
+          const vec2_t<T> w = g[walk.index_and_step(ctx, st.P, st.off, x, y, z)];
+          tile[m] = cplx_t<T>{w.x, w.y};
+        }
       }
       __syncthreads();
-      const gptr<T> cout = c0 + 2 * int64_t(v) * ctx.M;
+      if (pipelined && v + 1 < nvec) {  // next vector's tile, in flight during the sums
+        const gptr<const vec2_t<T>> g = g0 + int64_t(v + 1) * nftot;
+        int x = walk.x0, y = walk.y0, z = walk.z0;
 #pragma unroll
-      for (int k = 0; k < PT; ++k) {
+        for (int k = 0; k < MV_PF; ++k) {
+          const int m = threadIdx.x + k * MV_THREADS;
+          if (m < st.ntile) pre[k] = g[walk.index_and_step(ctx, st.P, st.off, x, y, z)];
+        }
+      }
+      const gptr<vec2_t<T>> cout = c0 + int64_t(v) * ctx.M;
+#pragma unroll
+      for (int k = 0; k < MV_PT; ++k) {
         if (!on[k]) continue;
         T ar = 0, ai = 0;
         if constexpr (DIM == 1) {
@@ -775,8 +885,10 @@ __device__ __forceinline__ void interp_unit_mv(const SpreadCtx<T> &ctx, const Ke
             ai = std::fma(pi, ker[k][2][e], ai);
           }
         }
-        cout[2 * int64_t(j[k])] = ar;
-        cout[2 * int64_t(j[k]) + 1] = ai;
+        vec2_t<T> o;
+        o.x = ar;
+        o.y = ai;
+        cout[j[k]] = o;
       }
     }
   }
@@ -877,7 +989,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 // Interpolation of many problems (see interp_batch): workgroup b takes work
 // unit b (a subproblem of one problem, over all vectors of the round).
 template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(BIN_THREADS)
+__global__ void __launch_bounds__(MV_THREADS)
     interp_batch_kernel(const InterpItem<T> *__restrict__ items, int nprob, KerParams<T> kp,
                         const cplx_t<T> *const *__restrict__ fw_base,
                         cplx_t<T> *const *__restrict__ c_base, int64_t c_vec_offset, int nvec,
@@ -903,8 +1015,9 @@ __global__ void __launch_bounds__(BIN_THREADS)
   StartTile<DIM, NS> st;
   if (!st.init_sub(it.ctx, int(u - it.unit_begin)) || st.npts == 0) return;
   interp_unit_mv<T, DIM, NS>(
-      it.ctx, kp, st, tile, to_global(reinterpret_cast<const T *>(fw_base[lo])), it.nftot,
-      to_global(reinterpret_cast<T *>(c_base[lo])) + 2 * c_vec_offset * int64_t(it.ctx.M), nvec);
+      it.ctx, kp, st, tile, to_global(reinterpret_cast<const vec2_t<T> *>(fw_base[lo])), it.nftot,
+      to_global(reinterpret_cast<vec2_t<T> *>(c_base[lo])) + c_vec_offset * int64_t(it.ctx.M),
+      nvec);
 }
 
 // Interpolant and coordinate derivatives, one lane per point (sorted order),
@@ -1104,8 +1217,8 @@ void interp_batch_ns(const InterpItem<T> *items, int nprob, const KerParams<T> &
   allow_lds(k, lds_bytes);
   if (total_units <= 0) return;
   if (total_units > INT32_MAX) fail(FINUFFT_ERR_MAXNALLOC, "interp_batch: too many work units");
-  k<<<unsigned(total_units), BIN_THREADS, lds_bytes, stream>>>(items, nprob, kp, fw_base, c_base,
-                                                               c_vec_offset, nvec, total_units);
+  k<<<unsigned(total_units), MV_THREADS, lds_bytes, stream>>>(items, nprob, kp, fw_base, c_base,
+                                                              c_vec_offset, nvec, total_units);
   launch_check("interp_batch_kernel");
 }
 

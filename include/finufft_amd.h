@@ -40,7 +40,7 @@ FINUFFT_EXPORT int cufinufftf_interp_grad(cufinufftf_plan d_plan, hipFloatComple
  * type-2 plans (cufinufft_plan or cufinufftf_plan as is_double says) with
  * points set, sharing dimension, ntransf, isign, tolerance and upsampfac,
  * made with gpu_method 0 or 2. Problems of equal fine-grid shape are run in
- * chunks whose grids fit arena_bytes (0: 96 MiB, a size meant to stay in the
+ * chunks whose grids fit arena_bytes (0: 192 MiB, a size meant to stay in the
  * GPU's last-level cache from stage to stage); the batch allocates one
  * chunk's grids. The plans must outlive the batch and keep their
  * points. execute takes host arrays of nplans device pointers: c[p] receives

@@ -139,3 +139,36 @@ def test_batch_chunks_with_remainder(torch_gpu):
     for p, fk, o in zip(plans, fks, outs):
         ref = to_host(p.execute(fk))
         assert nudft.relerr(to_host(o), ref) < 5e-6
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_batch_fused_transform_shapes(torch_gpu, dtype):
+    """2-D batches take the fused column / row transforms: rectangular and
+    large grids (one column or row per workgroup), 2,3,5-smooth lengths,
+    both mode orders and signs, against each problem's own plan (rocFFT)."""
+    from finufft_amd import Plan
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    r = rng(94)
+    real = np.float32 if dtype == "complex64" else np.float64
+    tol = 1e-5 if dtype == "complex64" else 1e-10
+    ntr = 2
+    for modeord, isign in ((0, 1), (1, -1)):
+        plans, fks, outs = [], [], []
+        for shape in ((37, 250), (512, 9), (1000, 1000), (37, 250)):
+            M = 20000
+            x = [to_dev(torch, r.uniform(-np.pi, np.pi, M).astype(real)) for _ in range(2)]
+            p = Plan(2, shape, n_trans=ntr, eps=tol, dtype=dtype, isign=isign, modeord=modeord)
+            p.setpts(*x)
+            plans.append(p)
+            fks.append(to_dev(torch, cnormal(r, (ntr,) + shape, np.dtype(dtype))))
+            outs.append(torch.empty((ntr, M), dtype=fks[-1].dtype, device="cuda"))
+        b = BatchPlan(plans)
+        assert b.info()["grid_shapes"] == 3
+        b.execute(fks, outs)
+        torch.cuda.synchronize()
+        bound = 2e-5 if dtype == "complex64" else 1e-12
+        for p, fk, o in zip(plans, fks, outs):
+            ref = to_host(p.execute(fk))
+            assert nudft.relerr(to_host(o), ref) < bound, (p.info()["nf"], modeord, isign)

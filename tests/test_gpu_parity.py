@@ -347,3 +347,30 @@ def test_interp_dense_bins(torch_gpu, maxsub, dtype):
     err = nudft.relerr(c, ref)
     print(maxsub, dtype, plan.info(), err)
     assert err < 25 * tol
+
+
+@pytest.mark.parametrize("ttype", [1, 2])
+def test_tile_wraps_twice(torch_gpu, ttype):
+    """Bins of 16 on a grid of 18 (N = 9, sigma 2): the second bin's padded
+    tile reaches past 2 nf, so its cells wrap around the periodic grid twice."""
+    from finufft_amd import Plan
+
+    torch = torch_gpu
+    r = rng(25)
+    shape, M, tol = (9, 9), 400, 1e-5
+    pts = uniform_pts(r, 2, M, np.float32)
+    plan = Plan(ttype, shape, eps=tol, dtype="complex64", gpu_method=2, gpu_binsizex=16,
+                gpu_binsizey=16)
+    plan.setpts(*[to_dev(torch, p) for p in pts])
+    info = plan.info()
+    assert info["nf"][:2] == (18, 18) and info["binsize"][:2] == (16, 16)
+    p64 = [p.astype(np.float64) for p in pts]
+    if ttype == 1:
+        c = cnormal(r, (M,), np.complex64)
+        got = to_host(plan.execute(to_dev(torch, c)))
+        ref = nudft.type1(p64, c, plan._isign, shape)
+    else:
+        fk = cnormal(r, shape, np.complex64)
+        got = to_host(plan.execute(to_dev(torch, fk)))
+        ref = nudft.type2(p64, fk, plan._isign)
+    assert nudft.relerr(got, ref) < 25 * tol
