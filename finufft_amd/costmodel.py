@@ -6,7 +6,9 @@ one output per option) predicts the log execute time of each option from
     dim, log2 N (modes), log2 M, log2 (M / N), log2 ntransf, -log10 tol,
     precision (0 fp32, 1 fp64), transform type (1 or 2)
 
-for the options (ARMS) upsampfac 2.0 / 1.25 x gpu_method 2 / 1. Training data
+for the options (ARMS): LDS subproblems (gpu_method 2, the default), global
+memory (gpu_method 1), all vectors of an execute in one pass
+(gpu_maxbatchsize 32) and upsampling factor 1.25. Training data
 come from sweeps of the library's own options on the GPU (`sweep`): every
 option is timed and its error against the FP64 direct sum measured at sampled
 outputs; an option whose error exceeds max(tol, 1.5 x the default option's
@@ -38,9 +40,9 @@ from . import _lib
 
 ARMS = [
     {"upsampfac": 2.0, "gpu_method": 2},
-    {"upsampfac": 1.25, "gpu_method": 2},
     {"upsampfac": 2.0, "gpu_method": 1},
-    {"upsampfac": 1.25, "gpu_method": 1},
+    {"upsampfac": 2.0, "gpu_method": 2, "gpu_maxbatchsize": 32},
+    {"upsampfac": 1.25, "gpu_method": 2},
 ]
 NIN, NHID, NOUT = 8, 32, 4
 NPARAM = NHID * NIN + NHID + NHID * NHID + NHID + NOUT * NHID + NOUT
@@ -139,7 +141,8 @@ def train(x, t, m, epochs=3000, lr=3e-3, seed=0, log=None):
 
 
 class CostModel:
-    def __init__(self, params, mean, std, lo, hi, rejected=None, meta=None):
+    def __init__(self, params, mean, std, lo, hi, rejected=None, meta=None, arms=None):
+        self.arms = [dict(a) for a in (arms or ARMS)]  # the options the network was trained on
         self.params = np.asarray(params, np.float32)
         self.mean = np.asarray(mean, np.float32)
         self.std = np.asarray(std, np.float32)
@@ -153,7 +156,7 @@ class CostModel:
     def load(cls, path=WEIGHTS):
         d = json.loads(Path(path).read_text())
         return cls(d["params"], d["mean"], d["std"], d["lo"], d["hi"], d.get("rejected"),
-                   d.get("meta"))
+                   d.get("meta"), d.get("arms"))
 
     def save(self, path):
         Path(path).write_text(json.dumps({
@@ -161,7 +164,7 @@ class CostModel:
             "mean": self.mean.tolist(), "std": self.std.tolist(),
             "lo": self.lo.tolist(), "hi": self.hi.tolist(),
             "rejected": self.rejected, "meta": self.meta,
-            "arms": ARMS,
+            "arms": self.arms,
         }, indent=1))
 
     def predict(self, feats):
@@ -196,8 +199,8 @@ class CostModel:
                 continue
             banned = set(self.rejected.get(problem_class(ttype, len(n_modes), tol, dtype), []))
             order = [int(i) for i in np.argsort(y)
-                     if int(i) not in banned and ARMS[int(i)]["gpu_method"] in methods]
-            out.append(dict(ARMS[order[0]]) if order else {})
+                     if int(i) not in banned and self.arms[int(i)]["gpu_method"] in methods]
+            out.append(dict(self.arms[order[0]]) if order else {})
         return out
 
 
@@ -269,8 +272,9 @@ def time_arm(ttype, n_modes, M, ntransf, tol, dtype, arm, reps=5, seed=0, err_sa
 
 
 def sweep_configs(seed=0, n=160):
-    """Problem configurations of the sweep: 2-D and 3-D, types 1 and 2, fp32,
-    sizes around the uneven small-problem and the 3-D workloads."""
+    """Problem configurations of the sweep: 2-D and 3-D, types 1 and 2, fp32
+    (three in four) and fp64, sizes around the uneven small-problem and the
+    3-D workloads."""
     r = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -283,8 +287,11 @@ def sweep_configs(seed=0, n=160):
             side = int(np.exp(r.uniform(np.log(12), np.log(96))))
             M = int(np.exp(r.uniform(np.log(1e3), np.log(2e6))))
         ntr = int(r.choice([1, 4, 32]))
-        tol = float(r.choice([1e-3, 1e-5, 1e-6]))
-        out.append((ttype, (side,) * dim, M, ntr, tol, "complex64"))
+        if r.uniform() < 0.75:
+            dtype, tol = "complex64", float(r.choice([1e-3, 1e-5, 1e-6]))
+        else:
+            dtype, tol = "complex128", float(r.choice([1e-6, 1e-9, 1e-12]))
+        out.append((ttype, (side,) * dim, M, ntr, tol, dtype))
     return out
 
 
@@ -328,19 +335,26 @@ def build_model(configs, log=print, epochs=3000):
 
 
 def evaluate(model, recs):
-    """Mean measured time of the model's choice vs the default option over
-    sweep records: (default_ms, chosen_ms, oracle_best_ms)."""
+    """Measured time of the model's choice vs the default option over sweep
+    records: (default_ms, chosen_ms, best_ms) summed over the records, and the
+    geometric means over records of default / chosen and default / best."""
     d = c = b = 0.0
+    lc = lb = 0.0
+    n = 0
     for r in recs:
         ms = r["ms"]
         if not (r["ok"][0] and math.isfinite(ms[0])):
             continue
         opts = model.choose(r["ttype"], r["n_modes"], r["M"], r["ntransf"], r["tol"], r["dtype"])
-        a = ARMS.index(opts) if opts in ARMS else 0
+        a = model.arms.index(opts) if opts in model.arms else 0
         chosen = ms[a] if (r["ok"][a] and math.isfinite(ms[a])) else ms[0]
         best = min(t for t, o in zip(ms, r["ok"]) if o and math.isfinite(t))
         d, c, b = d + ms[0], c + chosen, b + best
-    return d, c, b
+        lc += math.log(ms[0] / chosen)
+        lb += math.log(ms[0] / best)
+        n += 1
+    n = max(n, 1)
+    return d, c, b, math.exp(lc / n), math.exp(lb / n)
 
 
 def main(argv=None):
@@ -348,8 +362,8 @@ def main(argv=None):
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("cmd", choices=["train"])
     ap.add_argument("--out", default=str(WEIGHTS))
-    ap.add_argument("--configs", type=int, default=160)
-    ap.add_argument("--holdout", type=int, default=40)
+    ap.add_argument("--configs", type=int, default=240)
+    ap.add_argument("--holdout", type=int, default=60)
     ap.add_argument("--records", default="")
     args = ap.parse_args(argv)
 
@@ -361,13 +375,15 @@ def main(argv=None):
     model, recs = build_model(train_cfg, log=log)
     hold = sweep_configs(1, args.holdout)
     _, hrecs = build_model(hold, log=log, epochs=1)  # sweep only; its network is unused
-    d, c, b = evaluate(model, hrecs)
+    d, c, b, gc, gb = evaluate(model, hrecs)
     model.meta.update({"holdout_default_ms": d, "holdout_chosen_ms": c, "holdout_best_ms": b,
+                       "holdout_geomean_speedup": gc, "holdout_geomean_best": gb,
                        "sweep_s": time.time() - t0})
     model.save(args.out)
     if args.records:
         Path(args.records).write_text(json.dumps({"train": recs, "holdout": hrecs}))
-    log(f"held-out total time: default {d:.1f} ms, model {c:.1f} ms, best {b:.1f} ms")
+    log(f"held-out total time: default {d:.1f} ms, model {c:.1f} ms, best {b:.1f} ms; "
+        f"per-problem geometric-mean speedup: model {gc:.3f}, best {gb:.3f}")
     print(json.dumps(model.meta))
     return 0
 

@@ -95,8 +95,12 @@ def test_committed_model_gain(torch_gpu):
     cfgs = [(2, (256, 256), 50_000, 32, 1e-5, "complex64"),
             (2, (480, 480), 3_000, 32, 1e-5, "complex64"),
             (2, (48, 48), 40_000, 32, 1e-5, "complex64"),
-            (1, (64, 64, 64), 1_000_000, 1, 1e-6, "complex64")]
+            (1, (64, 64, 64), 1_000_000, 1, 1e-6, "complex64"),
+            (1, (40, 40, 40), 20_000, 32, 1e-3, "complex64"),
+            (2, (100, 100), 8_000, 32, 1e-6, "complex64"),
+            (1, (200, 200), 150_000, 4, 1e-9, "complex128")]
     t_def = t_mod = 0.0
+    log_gain = 0.0
     for i, c in enumerate(cfgs):
         opts = model.choose(*c) or costmodel.ARMS[0]
         ms0, e0 = costmodel.time_arm(*c, costmodel.ARMS[0], seed=100 + i)
@@ -105,5 +109,9 @@ def test_committed_model_gain(torch_gpu):
         assert e1 <= max(c[4], 1.5 * e0) * 1.5
         t_def += ms0
         t_mod += ms1
-    print("default", t_def, "model", t_mod, "gain", t_def / t_mod)
+        log_gain += math.log(ms0 / ms1)
+    gmean = math.exp(log_gain / len(cfgs))
+    print("default", t_def, "model", t_mod, "total gain", t_def / t_mod, "per-problem gain", gmean)
     assert t_mod <= 1.1 * t_def
+    # the gain: many-vector problems of these sizes run faster in one pass
+    assert gmean > 1.05
