@@ -78,7 +78,7 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
   fused_ = dim_ == 2;
   for (size_t gi = 0; gi < groups_.size() && fused_; ++gi) {
     Group &g = *groups_[gi];
-    fused_ = fused_fft_shape<T>(g.nf, g.cw, g.lcw, g.rw, g.lds_col, g.lds_row);
+    fused_ = fused_fft_shape<T>(g.nf, g.cw, g.lcw, g.rw, g.lrw, g.lds_col, g.lds_row);
   }
 
   // Chunks: consecutive problems of one group, as many as fit the budget.
@@ -181,6 +181,7 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
         f.cw = g.cw;
         f.lcw = g.lcw;
         f.rw = g.rw;
+        f.lrw = g.lrw;
         f.ncol = (f.ms[0] + f.cw - 1) / f.cw;
         f.nrow = (f.nf[1] + f.rw - 1) / f.rw;
         f.nmodes = p.nmodes_;

@@ -66,7 +66,7 @@ class BatchPlan {
     int per_chunk = 1;  // problems per full chunk
     FftPlan fft_full, fft_rest;  // unused on the fused 2-D path
     dbuf<T> tw[2], rker[2];      // twiddles and reciprocal kernel series (fused 2-D path)
-    int cw = 1, lcw = 0, rw = 1;  // fused 2-D path: columns / rows per workgroup
+    int cw = 1, lcw = 0, rw = 1, lrw = 0;  // fused 2-D path: columns / rows per workgroup
     size_t lds_col = 0, lds_row = 0;
   };
   struct Chunk {

@@ -35,7 +35,7 @@ bool radix_plan(int n, RadixPlan &out) {
 }
 
 template <typename T>
-bool fused_fft_shape(const int nf[2], int &cw, int &lcw, int &rw, size_t &lds_col,
+bool fused_fft_shape(const int nf[2], int &cw, int &lcw, int &rw, int &lrw, size_t &lds_col,
                      size_t &lds_row) {
   const int Ec = fft_col_elems<T>(), Er = fft_row_elems<T>();
   RadixPlan rp;
@@ -47,7 +47,12 @@ bool fused_fft_shape(const int nf[2], int &cw, int &lcw, int &rw, size_t &lds_co
     cw *= 2;
     ++lcw;
   }
-  rw = std::max(1, std::min(16, Er / nf[0]));
+  rw = 1;
+  lrw = 0;
+  while (rw < 16 && 2 * rw * nf[0] <= Er) {
+    rw *= 2;
+    ++lrw;
+  }
   const size_t cell = 2 * sizeof(T);
   lds_col = (2 * size_t(cw) * nf[1] + FFT_TW_LO + FFT_TW_HI) * cell;
   lds_row = (2 * size_t(rw) * nf[0] + FFT_TW_LO + FFT_TW_HI) * cell;
@@ -327,17 +332,23 @@ __global__ void __launch_bounds__(FFT_THREADS)
   v2<T> *TW = B + rw * nf0;
   stage_twiddles<T>(TW, to_global(reinterpret_cast<const v2<T> *>(it.tw[0])), nf0);
   const float rn0 = 1.0f / float(nf0);
-  // element (x, r) at r * nf0 + x: the mode columns of row row0 + r
+  const int lrw = it.lrw;
+  // element (x, r) at x * rw + r (rows interleaved, so that the butterflies
+  // index with shifts): the mode columns of row row0 + r, read row by row
   for (int e = threadIdx.x; e < rw * nf0; e += FFT_THREADS) {
     int x;
     const int r = divmod24(e, nf0, rn0, x);
     v2<T> val = {T(0), T(0)};
     if (r < nr && (x < h0 || x >= nf0 - m0 / 2)) val = fw[e];
-    A[e] = val;
+    A[(x << lrw) + r] = val;
   }
   __syncthreads();
-  const v2<T> *res = lds_fft<T>(A, B, nf0, rw, 0, false, it.rp[0], TW, sign);
-  for (int e = threadIdx.x; e < nr * nf0; e += FFT_THREADS) fw[e] = res[e];
+  const v2<T> *res = lds_fft<T>(A, B, nf0, rw, lrw, true, it.rp[0], TW, sign);
+  for (int e = threadIdx.x; e < nr * nf0; e += FFT_THREADS) {
+    int x;
+    const int r = divmod24(e, nf0, rn0, x);
+    fw[e] = res[(x << lrw) + r];
+  }
 }
 
 }  // namespace
@@ -376,7 +387,8 @@ void fused_fft_rows(const FusedFftItem<T> *items, cplx_t<T> *const *fw_base, int
 }
 
 #define FA_INST(T)                                                                            \
-  template bool fused_fft_shape<T>(const int[2], int &, int &, int &, size_t &, size_t &);   \
+  template bool fused_fft_shape<T>(const int[2], int &, int &, int &, int &, size_t &,       \
+                                   size_t &);                                                 \
   template void fused_fft_prepare<T>();                                                       \
   template void fft_twiddles<T>(int, std::vector<T> &);                                      \
   template void fused_fft_cols<T>(const FusedFftItem<T> *, const cplx_t<T> *const *,         \

@@ -61,7 +61,7 @@ struct FusedFftItem {
   int ms[2], nf[2];
   int modeord;
   int cw, lcw, ncol;  // lcw = log2 cw
-  int rw, nrow;
+  int rw, lrw, nrow;  // lrw = log2 rw
   RadixPlan rp[2];
   const T *rker[2];  // reciprocal kernel Fourier coefficients per axis
   const T *tw[2];   // (cos, sin)(2 pi j / nf_d), j < nf_d, as pairs
@@ -73,7 +73,7 @@ struct FusedFftItem {
 // at most FFT_MAX_N), its workgroup shapes and the LDS bytes of the column
 // and row launches.
 template <typename T>
-bool fused_fft_shape(const int nf[2], int &cw, int &lcw, int &rw, size_t &lds_col,
+bool fused_fft_shape(const int nf[2], int &cw, int &lcw, int &rw, int &lrw, size_t &lds_col,
                      size_t &lds_row);
 
 // Twiddle table of length n (2n values: cos, sin of 2 pi j / n), computed in
