@@ -537,10 +537,12 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
   double h[3] = {1, 1, 1}, gam[3] = {1, 1, 1};
   int64_t nf123[3] = {1, 1, 1};
   for (int d = 0; d < dim_; ++d) {
-    double lo, hi;
-    minmax<T>(src[d], M, &lo, &hi, stream_);
+    // an empty set of sources or targets has zero width and centre
+    double lo = 0, hi = 0;
+    if (M > 0) minmax<T>(src[d], M, &lo, &hi, stream_);
     const auto [X, Cc] = widcen(lo, hi);
-    minmax<T>(trg[d], N, &lo, &hi, stream_);
+    lo = hi = 0;
+    if (N > 0) minmax<T>(trg[d], N, &lo, &hi, stream_);
     const auto [S, D] = widcen(lo, hi);
     // The type-3 parameters live in the plan's precision, as in the reference.
     t3C_[d] = double(T(Cc));
