@@ -690,29 +690,14 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
   }
 }
 
-// Interpolation of one subproblem over nvec vectors (the fused launch over
-// many problems). The points are taken in chunks of up to two per thread;
-// each point's kernel weights and tile offsets stay in registers while the
-// tile of every vector in turn is staged and applied, so a point's kernel is
-// evaluated once per round rather than once per vector. Global arrays are
-// accessed through global-typed pointers and the Horner table through a
-// constant-typed one (scalar loads).
-#ifndef FA_MV_THREADS
-#define FA_MV_THREADS 512
-#endif
-#ifndef FA_MV_PT
-#define FA_MV_PT 2
-#endif
-#ifndef FA_MV_PF
-#define FA_MV_PF 6
-#endif
 // Workgroup size of the multi-vector interpolation, points per thread per
 // chunk, and tile cells per thread held in registers while the previous
 // vector is interpolated (tiles larger than MV_PF * MV_THREADS cells are
-// staged without the overlap).
-constexpr int MV_THREADS = FA_MV_THREADS;
-constexpr int MV_PT = FA_MV_PT;
-constexpr int MV_PF = FA_MV_PF;
+// staged without the overlap). Chosen by a sweep on the batched 2-D workload
+// (256 / 512 threads, 1 / 2 points, 1 / 6 / 12 cells per thread).
+constexpr int MV_THREADS = 512;
+constexpr int MV_PT = 2;
+constexpr int MV_PF = 6;
 
 // Walks a tile in steps of MV_THREADS cells: each thread's first cell and
 // the step in tile coordinates are found once per unit, so that the
