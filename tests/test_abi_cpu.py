@@ -83,6 +83,11 @@ def test_error_codes_before_device():
     assert _makeplanf(1, 2, N, opts=o) == 7  # FINUFFT_ERR_UPSAMPFAC_TOO_SMALL
     o.gpu_kerevalmeth = 1
     assert _makeplanf(1, 2, N, opts=o) == 8  # FINUFFT_ERR_HORNER_WRONG_BETA
+    o = _lib.default_opts()
+    o.gpu_method = 3
+    assert _makeplanf(2, 2, N, opts=o) == 17  # FINUFFT_ERR_METHOD_NOTVALID: method 3 only spreads
+    o.gpu_method = 5
+    assert _makeplanf(1, 2, N, opts=o) == 19  # FINUFFT_ERR_INSUFFICIENT_SHMEM (heuristics.cu:440)
     lib = _lib.lib()
     assert lib.cufinufft_destroy(None) == 16  # FINUFFT_ERR_PLAN_NOTVALID
     assert lib.cufinufftf_destroy(None) == 16
