@@ -273,7 +273,8 @@ def main(argv=None):
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(the library runs one plan on one GPU; it shards nothing across GPUs)",
+              file=sys.stderr)
         return 2
 
     import torch

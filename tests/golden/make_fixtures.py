@@ -1,7 +1,7 @@
 """Writes the golden fixtures of tests/golden/ from data held in the reference.
 
-Run in a container that has the reference checked out (read-only) at
-/root/reference; the GPU box never runs this. Outputs are data only:
+Run where the reference is checked out (read-only) at /root/reference; the
+tests only read its outputs. Outputs are data only:
 
 * horner_sigma2.json / horner_sigma125.json: the Horner coefficient tables of
   include/cufinufft/contrib/ker_horner_allw_loop.inc and
