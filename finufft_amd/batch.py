@@ -12,6 +12,8 @@ from __future__ import annotations
 
 from ctypes import byref, c_int64, c_void_p
 
+import torch
+
 from . import _lib
 from ._plan import FinufftError, Plan
 
@@ -27,6 +29,17 @@ class BatchPlan:
         if any(not isinstance(p, Plan) for p in self._plans):
             raise TypeError("plans must be finufft_amd.Plan objects")
         self._double = self._plans[0]._double
+        if any(p._double != self._double for p in self._plans):
+            raise TypeError("plans must share the precision")
+        self._dtype = getattr(torch, self._plans[0].dtype)
+        # element counts of every problem's modes and values (n_trans vectors)
+        self._fk_sizes, self._out_sizes = [], []
+        for p in self._plans:
+            modes = 1
+            for m in p.n_modes:
+                modes *= int(m)
+            self._fk_sizes.append(p.n_trans * modes)
+            self._out_sizes.append(p.n_trans * p._nj)
         handles = (c_void_p * len(self._plans))(*[p.handle for p in self._plans])
         h = c_void_p(0)
         ier = _lib.lib().finufft_amd_batch_makeplan(int(self._double), len(self._plans), handles,
@@ -41,6 +54,19 @@ class BatchPlan:
         n = len(self._plans)
         if len(fk_list) != n or len(out_list) != n:
             raise ValueError("one input and one output per problem")
+        # The library reads and writes through these pointers without knowing
+        # the sizes: every array is checked against its plan (kept short, it
+        # runs per problem on every execute).
+        want = self._dtype
+        for i, (f, o, nf, no) in enumerate(zip(fk_list, out_list, self._fk_sizes,
+                                              self._out_sizes)):
+            if not (f.dtype == want and o.dtype == want and f.is_cuda and o.is_cuda
+                    and f.is_contiguous() and o.is_contiguous()):
+                raise TypeError(f"problem {i}: inputs and outputs must be contiguous "
+                                f"{want} tensors on the GPU")
+            if f.numel() != nf or o.numel() != no:
+                raise ValueError(f"problem {i}: {f.numel()} modes and {o.numel()} values "
+                                 f"given, its plan needs {nf} and {no}")
         c = (c_void_p * n)(*[o.data_ptr() for o in out_list])
         fk = (c_void_p * n)(*[f.data_ptr() for f in fk_list])
         ier = _lib.lib().finufft_amd_batch_execute(self._h, c, fk)

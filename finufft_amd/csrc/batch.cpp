@@ -31,6 +31,9 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
            "evaluation method)");
     if (p->opts_.gpu_device_id != p0.opts_.gpu_device_id)
       fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: plans must share the device");
+    // setpts builds the subproblem table that the batch reads below
+    if (p->substart_.get() == nullptr)
+      fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: every plan must have its points set");
   }
   device_guard dg(p0.opts_.gpu_device_id);
   const int n = nprob();
