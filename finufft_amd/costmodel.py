@@ -77,7 +77,15 @@ def _torch():
     return torch
 
 
-def _ptr(t):
+def _ptr(t, n=None, name="array"):
+    """Device pointer of a contiguous float32 GPU tensor of n elements (the
+    kernels index it without knowing its size)."""
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.dtype == torch.float32 and t.is_cuda
+            and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous float32 tensor on the GPU")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{name} has {t.numel()} elements, {n} expected")
     return t.data_ptr()
 
 
@@ -85,9 +93,10 @@ def gpu_forward(params, x):
     """y[n][NOUT] from x[n][NIN] with the library's forward kernel (torch
     float32 tensors on the GPU)."""
     torch = _torch()
-    y = torch.empty((x.shape[0], NOUT), dtype=torch.float32, device=x.device)
-    ier = _lib.lib().finufft_amd_mlp_forward(_ptr(params), int(x.shape[0]), _ptr(x), _ptr(y),
-                                             None)
+    n = int(x.shape[0])
+    y = torch.empty((n, NOUT), dtype=torch.float32, device=x.device)
+    ier = _lib.lib().finufft_amd_mlp_forward(_ptr(params, NPARAM, "params"), n,
+                                             _ptr(x, n * NIN, "x"), _ptr(y), None)
     if ier != 0:
         raise RuntimeError(f"finufft_amd_mlp_forward failed ({ier})")
     return y
@@ -96,10 +105,12 @@ def gpu_forward(params, x):
 def gpu_grad(params, x, t, m):
     """(loss, grad) of the masked squared error with the library's kernel."""
     torch = _torch()
+    n = int(x.shape[0])
     g = torch.zeros(NPARAM, dtype=torch.float32, device=x.device)
     loss = torch.zeros(1, dtype=torch.float32, device=x.device)
-    ier = _lib.lib().finufft_amd_mlp_grad(_ptr(params), int(x.shape[0]), _ptr(x), _ptr(t),
-                                          _ptr(m), _ptr(g), _ptr(loss), None)
+    ier = _lib.lib().finufft_amd_mlp_grad(_ptr(params, NPARAM, "params"), n,
+                                          _ptr(x, n * NIN, "x"), _ptr(t, n * NOUT, "target"),
+                                          _ptr(m, n * NOUT, "mask"), _ptr(g), _ptr(loss), None)
     if ier != 0:
         raise RuntimeError(f"finufft_amd_mlp_grad failed ({ier})")
     return loss, g

@@ -1,6 +1,8 @@
-"""Host-side argument checks of finufft_amd.batch.BatchPlan.execute (no GPU):
-the library writes through the pointers it is given without knowing their
-sizes, so every array is checked against its plan before the call."""
+"""Host-side argument checks (no GPU) of the Python wrappers whose library
+calls take bare device pointers: BatchPlan.execute (the library writes
+through the pointers it is given without knowing their sizes, so every array
+is checked against its plan before the call) and the cost model's network
+kernels."""
 
 import pytest
 import torch
@@ -53,3 +55,14 @@ def test_sizes_against_the_plans(fk, out):
     b = _bare_batch([8, 8], [5, 6])
     with pytest.raises(ValueError, match="problem 1: .* its plan needs 8 and 6"):
         b.execute([FakeTensor(n) for n in fk], [FakeTensor(n) for n in out])
+
+
+def test_costmodel_pointer_checks():
+    from finufft_amd import costmodel
+
+    with pytest.raises(TypeError, match="float32 tensor on the GPU"):
+        costmodel._ptr(torch.zeros(costmodel.NPARAM), costmodel.NPARAM, "params")
+    with pytest.raises(TypeError):
+        costmodel._ptr(torch.zeros(costmodel.NPARAM, dtype=torch.float64))
+    with pytest.raises(TypeError):
+        costmodel._ptr(None)
