@@ -180,7 +180,17 @@ class Plan:
             if output:
                 raise ValueError(f"Argument `{name}` must be contiguous")
             t = t.contiguous()
+            self._hold_for_plan_stream(t)
         return t
+
+    def _hold_for_plan_stream(self, t):
+        """A temporary copy read by the library asynchronously on the plan's
+        stream: when that is not torch's current stream, torch's allocator is
+        told not to reuse the copy's memory before the plan's stream has
+        reached this point."""
+        ptr = self._opts.gpu_stream or 0
+        if ptr and ptr != torch.cuda.current_stream(t.device).cuda_stream:
+            t.record_stream(torch.cuda.ExternalStream(ptr, device=t.device))
 
     def setpts(self, x, y=None, z=None, s=None, t=None, u=None):
         """Set the nonuniform points (and the type-3 targets s, t, u)."""
