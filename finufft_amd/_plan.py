@@ -184,10 +184,11 @@ class Plan:
         return t
 
     def _hold_for_plan_stream(self, t):
-        """A temporary copy read by the library asynchronously on the plan's
+        """A tensor allocated here (a contiguous copy of an input, or an
+        output) that the library reads or writes asynchronously on the plan's
         stream: when that is not torch's current stream, torch's allocator is
-        told not to reuse the copy's memory before the plan's stream has
-        reached this point."""
+        told not to reuse its memory before the plan's stream has reached
+        this point."""
         ptr = self._opts.gpu_stream or 0
         if ptr and ptr != torch.cuda.current_stream(t.device).cuda_stream:
             t.record_stream(torch.cuda.ExternalStream(ptr, device=t.device))
@@ -243,6 +244,7 @@ class Plan:
                              f"{(n, *in_shape) if n > 1 else in_shape}")
         if out is None:
             out = torch.empty(batch + out_shape, dtype=data.dtype, device=data.device)
+            self._hold_for_plan_stream(out)
         else:
             out = self._check(out, "out", False, output=True)
             if out.numel() != n * int(torch.tensor(out_shape).prod()):
@@ -289,6 +291,9 @@ class Plan:
             raise ValueError(f"Invalid fk shape {tuple(fk.shape)}")
         c = torch.empty(batch + (self._nj,), dtype=fk.dtype, device=fk.device) if value else None
         dc = torch.empty((self._dim,) + batch + (self._nj,), dtype=fk.dtype, device=fk.device)
+        for t in (c, dc):
+            if t is not None:
+                self._hold_for_plan_stream(t)
         fn = f"{self._pref}_interp_grad"
         ier = getattr(_lib.lib(), fn)(self._plan, c.data_ptr() if value else None,
                                       dc.data_ptr(), fk.data_ptr())
