@@ -81,6 +81,7 @@ class Plan:
         self._n_trans, self._isign, self._eps = int(n_trans), int(isign), float(eps)
         self._refs = []
         self._nj = self._nk = 0
+        self._pts_gen = 0  # counts setpts calls (autograd checks which points are set)
 
         opts = _lib.default_opts()
         stream_given = False
@@ -216,6 +217,7 @@ class Plan:
         self._refs = [p for p in pts + trg if p is not None]
         ier = getattr(_lib.lib(), f"{self._pref}_setpts")(self._plan, M, *cp, N, *ct)
         self._nj, self._nk = M, N
+        self._pts_gen += 1
         if ier != 0:
             raise FinufftError("setpts", ier)
 

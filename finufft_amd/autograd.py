@@ -13,7 +13,9 @@ semantics are those of the CPU library's execute_adjoint
 (include/finufft/execute.hpp:318-357).
 
 Points are set on the plan for every forward call (they may change between
-iterations); the backward pass reuses that plan and its sort.
+iterations); the backward pass reuses that plan and its sort, and sets the
+forward call's points again when another call has replaced them since (two
+forward calls of one module before a backward).
 """
 
 from __future__ import annotations
@@ -27,18 +29,29 @@ def _sum_batch(t, nbatch):
     return t.sum(dim=0) if nbatch else t
 
 
+def _points_of(ctx):
+    """The plan with the points of ctx's forward call set on it, and the
+    forward's data tensor."""
+    data, *pts = ctx.saved_tensors
+    plan = ctx.plan
+    if plan._pts_gen != ctx.pts_gen:
+        plan.setpts(*[p.detach() for p in pts])
+        ctx.pts_gen = plan._pts_gen
+    return plan, data
+
+
 class _Type2(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f, plan, *pts):
         c = plan.execute(f)
         ctx.plan = plan
-        ctx.save_for_backward(f)
+        ctx.pts_gen = plan._pts_gen
+        ctx.save_for_backward(f, *pts)
         return c
 
     @staticmethod
     def backward(ctx, g):
-        (f,) = ctx.saved_tensors
-        plan = ctx.plan
+        plan, f = _points_of(ctx)
         g = g.contiguous()
         grad_f = plan.execute_adjoint(g) if ctx.needs_input_grad[0] else None
         grads = [None] * (len(ctx.needs_input_grad) - 2)
@@ -56,13 +69,13 @@ class _Type1(torch.autograd.Function):
     def forward(ctx, c, plan, *pts):
         f = plan.execute(c)
         ctx.plan = plan
-        ctx.save_for_backward(c)
+        ctx.pts_gen = plan._pts_gen
+        ctx.save_for_backward(c, *pts)
         return f
 
     @staticmethod
     def backward(ctx, g):
-        (c,) = ctx.saved_tensors
-        plan = ctx.plan
+        plan, c = _points_of(ctx)
         g = g.contiguous()
         need_x = any(ctx.needs_input_grad[2:])
         grads = [None] * (len(ctx.needs_input_grad) - 2)

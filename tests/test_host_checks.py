@@ -66,3 +66,48 @@ def test_costmodel_pointer_checks():
         costmodel._ptr(torch.zeros(costmodel.NPARAM, dtype=torch.float64))
     with pytest.raises(TypeError):
         costmodel._ptr(None)
+
+
+class _FakePlan:
+    """Records which points are set when the adjoint runs (no GPU)."""
+
+    def __init__(self):
+        self._pts_gen = 0
+        self.current = None
+        self.seen = []
+
+    def setpts(self, *pts):
+        self._pts_gen += 1
+        self.current = pts[0].clone()
+
+    def execute(self, data):
+        return data.clone()
+
+    def execute_adjoint(self, g):
+        self.seen.append(self.current.clone())
+        return g.clone()
+
+
+@pytest.mark.parametrize("kind", ["type1", "type2"])
+def test_autograd_two_forwards_one_backward(kind):
+    """Two forward calls on one plan before a backward: each backward runs
+    with the points of its own forward call."""
+    from finufft_amd import autograd
+
+    fn = autograd._Type1 if kind == "type1" else autograd._Type2
+    plan = _FakePlan()
+    x1, x2 = torch.linspace(0, 1, 5), torch.linspace(2, 3, 5)
+    d1 = torch.ones(5, dtype=torch.complex64, requires_grad=True)
+    d2 = torch.ones(5, dtype=torch.complex64, requires_grad=True)
+    plan.setpts(x1)
+    y1 = fn.apply(d1, plan, x1)
+    plan.setpts(x2)
+    y2 = fn.apply(d2, plan, x2)
+    (y1.abs().sum() + 2 * y2.abs().sum()).backward()
+    assert len(plan.seen) == 2
+    # the graph runs the second call's backward first (its points are set),
+    # then the first call's, which sets x1 again
+    got = sorted(plan.seen, key=lambda t: float(t[0]))
+    assert torch.equal(got[0], x1) and torch.equal(got[1], x2)
+    assert torch.allclose(d1.grad.real, torch.ones(5)) and torch.allclose(d2.grad.real,
+                                                                         2 * torch.ones(5))
