@@ -13,6 +13,8 @@
 #include <cstdint>
 #include <limits>
 #include <memory>
+#include <mutex>
+#include <unordered_set>
 
 #include "batch.hpp"
 #include "kernel_math.hpp"
@@ -43,10 +45,47 @@ bool invalid_modes(int type, int dim, const int64_t *m64, int m32[3]) {
   return tot > I32MAX;
 }
 
+// Live handles of one kind (fp32 plans, fp64 plans, batches). Every entry
+// point looks its handle up before using it, so that a destroyed handle, a
+// plan of the other precision or a stray pointer is refused with
+// FINUFFT_ERR_PLAN_NOTVALID instead of being dereferenced.
+template <typename K>
+struct Live {
+  std::mutex mu;
+  std::unordered_set<const void *> set;
+
+  static Live &get() {
+    static Live l;
+    return l;
+  }
+  void add(const void *p) {
+    std::lock_guard<std::mutex> lk(mu);
+    set.insert(p);
+  }
+  bool has(const void *p) {
+    std::lock_guard<std::mutex> lk(mu);
+    return set.count(p) != 0;
+  }
+  bool remove(const void *p) {
+    std::lock_guard<std::mutex> lk(mu);
+    return set.erase(p) != 0;
+  }
+};
+
 template <typename T>
 Plan<T> *as_plan(void *p) {
   if (!p) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null plan");
+  if (!Live<Plan<T>>::get().has(p))
+    fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "not a live plan of this precision");
   return static_cast<Plan<T> *>(p);
+}
+
+template <typename T>
+void destroy_plan(void *p) {
+  if (!p) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null plan");
+  if (!Live<Plan<T>>::get().remove(p))
+    fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "not a live plan of this precision");
+  delete static_cast<Plan<T> *>(p);
 }
 
 template <typename T, typename P>
@@ -63,7 +102,9 @@ int makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntr, T ep
     else
       cufinufft_default_opts(&o);
     if (!out) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "null plan pointer");
-    *out = reinterpret_cast<P>(new Plan<T>(type, dim, m32, iflag, ntr, eps, o));
+    auto plan = std::make_unique<Plan<T>>(type, dim, m32, iflag, ntr, eps, o);
+    Live<Plan<T>>::get().add(plan.get());
+    *out = reinterpret_cast<P>(plan.release());
   });
 }
 
@@ -113,6 +154,7 @@ struct BatchHandle {
 };
 BatchHandle *as_batch(void *b) {
   if (!b) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null batch");
+  if (!Live<BatchHandle>::get().has(b)) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "not a live batch");
   return static_cast<BatchHandle *>(b);
 }
 template <typename T>
@@ -177,10 +219,10 @@ int cufinufftf_execute(cufinufftf_plan p, hipFloatComplex *c, hipFloatComplex *f
 }
 
 int cufinufft_destroy(cufinufft_plan p) {
-  return safe_call([&] { delete as_plan<double>(p); });
+  return safe_call([&] { destroy_plan<double>(p); });
 }
 int cufinufftf_destroy(cufinufftf_plan p) {
-  return safe_call([&] { delete as_plan<float>(p); });
+  return safe_call([&] { destroy_plan<float>(p); });
 }
 
 int cufinufft_execute_adjoint(cufinufft_plan p, hipDoubleComplex *c, hipDoubleComplex *fk) {
@@ -211,6 +253,7 @@ int finufft_amd_batch_makeplan(int is_double, int nplans, void *const *plans, si
       h->d = make_batch<double>(nplans, plans, arena_bytes);
     else
       h->f = make_batch<float>(nplans, plans, arena_bytes);
+    Live<BatchHandle>::get().add(h.get());
     *out = h.release();
   });
 }
@@ -227,7 +270,12 @@ int finufft_amd_batch_execute(finufft_amd_batch batch, void *const *c, void *con
 }
 
 int finufft_amd_batch_destroy(finufft_amd_batch batch) {
-  return safe_call([&] { delete as_batch(batch); });
+  return safe_call([&] {
+    if (!batch) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null batch");
+    if (!Live<BatchHandle>::get().remove(batch))
+      fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "not a live batch");
+    delete static_cast<BatchHandle *>(batch);
+  });
 }
 
 int finufft_amd_batch_info(finufft_amd_batch batch, int64_t *iout) {

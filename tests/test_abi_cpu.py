@@ -93,6 +93,18 @@ def test_error_codes_before_device():
     assert lib.cufinufftf_destroy(None) == 16
     assert lib.cufinufftf_execute(None, None, None) == 16
     assert lib.cufinufftf_setpts(None, 1, None, None, None, 0, None, None, None) == 16
+    # a pointer that is not a live plan (never made, destroyed, or of the other
+    # precision) is refused without being dereferenced
+    stray = c_void_p(0x1000)
+    assert lib.cufinufftf_execute(stray, None, None) == 16
+    assert lib.cufinufft_execute(stray, None, None) == 16
+    assert lib.cufinufftf_setpts(stray, 1, None, None, None, 0, None, None, None) == 16
+    assert lib.cufinufftf_destroy(stray) == 16
+    assert lib.cufinufft_destroy(stray) == 16
+    iout = (c_int64 * 16)()
+    assert lib.finufft_amd_plan_info(stray, 0, iout, None) == 16
+    assert lib.finufft_amd_batch_execute(stray, None, None) == 16
+    assert lib.finufft_amd_batch_destroy(stray) == 16
 
 
 def test_simple_interface_argument_errors():
