@@ -519,6 +519,8 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
                      const T *u) {
   device_guard dg(opts_.gpu_device_id);
   if (M < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "negative number of points");
+  if (M > 0 && (x == nullptr || (dim_ > 1 && y == nullptr) || (dim_ > 2 && z == nullptr)))
+    fail(FINUFFT_ERR_INVALID_ARGUMENT, "null coordinate array");
   if (type_ != 3) {
     M_ = M;
     sort_points(x, y, z);
@@ -708,8 +710,18 @@ void Plan<T>::exec_type3(C *d_c, C *d_fk) {
 }
 
 template <typename T>
+void Plan<T>::check_data(const C *c, const C *fk) const {
+  // Null data arrays are refused where a kernel would read or write them.
+  if (type_ == 3 && !inner_)
+    fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: setpts must come before execute");
+  if (M_ > 0 && c == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "null c array");
+  if ((type_ != 3 || Nk_ > 0) && fk == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "null fk array");
+}
+
+template <typename T>
 void Plan<T>::execute(C *c, C *fk) {
   device_guard dg(opts_.gpu_device_id);
+  check_data(c, fk);
   switch (type_) {
     case 1: return exec_type1(c, fk, iflag_, ntransf_);
     case 2: return exec_type2(c, fk, iflag_, ntransf_);
@@ -720,6 +732,7 @@ void Plan<T>::execute(C *c, C *fk) {
 template <typename T>
 void Plan<T>::execute_adjoint(C *c, C *fk) {
   device_guard dg(opts_.gpu_device_id);
+  check_data(c, fk);
   if (type_ == 1) return exec_type2(c, fk, -iflag_, ntransf_);
   if (type_ == 2) return exec_type1(c, fk, -iflag_, ntransf_);
   fail(FINUFFT_ERR_TYPE_NOTVALID, "adjoint is provided for types 1 and 2");
@@ -733,6 +746,7 @@ void Plan<T>::interp_grad(C *d_c, C *d_dc, C *d_fk) {
   if (type_ == 3) fail(FINUFFT_ERR_TYPE_NOTVALID, "interp_grad is provided for types 1 and 2");
   if (spread_only_) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad needs the full transform");
   if (d_dc == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad: dc is null");
+  if (d_fk == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad: fk is null");
   const int fft_sign = type_ == 2 ? iflag_ : -iflag_;
   const SpreadCtx<T> ctx = spread_ctx();
   const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};

@@ -78,6 +78,7 @@ class Plan {
   void setup_bins();
   void sort_points(const T *x, const T *y, const T *z);
   SpreadCtx<T> spread_ctx() const;
+  void check_data(const C *c, const C *fk) const;
   void exec_type1(C *c, C *fk, int fft_sign, int ntr);
   void exec_type2(C *c, C *fk, int fft_sign, int ntr);
   void exec_type3(C *c, C *fk);
