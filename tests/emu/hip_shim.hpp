@@ -7,6 +7,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <barrier>
 #include <cmath>
 #include <cstdint>
@@ -31,8 +32,41 @@ struct emu_dim3 {
 };
 inline thread_local emu_dim3 threadIdx, blockIdx;
 inline std::barrier<> *emu_block_barrier = nullptr;
+inline std::barrier<> *emu_wave_barriers[16] = {};
 inline unsigned char *emu_lds = nullptr;
 inline void __syncthreads() { emu_block_barrier->arrive_and_wait(); }
+
+// A wave's lanes run as separate threads: the points where the device code
+// relies on the wave executing together (wave barriers, cross-lane reads)
+// become barriers over the wave's 64 threads. Fences have no work left to do.
+inline void __builtin_amdgcn_wave_barrier() { emu_wave_barriers[threadIdx.x >> 6]->arrive_and_wait(); }
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+inline uint64_t emu_lane_slots[16][64];
+template <typename V>
+inline V __builtin_amdgcn_readlane(V v, int k) {
+  static_assert(sizeof(V) <= 8, "readlane of a 32- or 64-bit value");
+  const unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t bits = 0;
+  std::memcpy(&bits, &v, sizeof(V));
+  emu_wave_barriers[w]->arrive_and_wait();
+  emu_lane_slots[w][lane] = bits;
+  emu_wave_barriers[w]->arrive_and_wait();
+  bits = emu_lane_slots[w][k];
+  emu_wave_barriers[w]->arrive_and_wait();
+  V r;
+  std::memcpy(&r, &bits, sizeof(V));
+  return r;
+}
+
+// Atomic adds (LDS or global) as host atomics.
+template <typename V>
+inline V atomicAdd(V *p, V v) {
+  return std::atomic_ref<V>(*p).fetch_add(v);
+}
+template <typename V>
+inline V unsafeAtomicAdd(V *p, V v) {
+  return std::atomic_ref<V>(*p).fetch_add(v);
+}
 
 using std::max;
 using std::min;
@@ -45,6 +79,7 @@ struct double2 {
 };
 using hipStream_t = void *;
 
+#ifndef EMU_NO_FA_TYPES  // the sources that define these themselves set it
 namespace fa {
 template <typename T>
 struct cplx_of;
@@ -66,27 +101,42 @@ inline P *to_global(P *p) {
 }
 constexpr double PI = 3.14159265358979323846;
 }  // namespace fa
+#endif
 
-// Runs blocks [0, nblocks) of a kernel one after another, nthreads OS threads
-// each, with a fresh LDS block of lds_bytes filled with 0xff (so that a read
-// before a write shows up as NaN).
+// Runs the blocks (x, y) of a grid nblocks x nblocks_y one after another on
+// one team of nthreads OS threads (whole waves when nthreads >= 64). The
+// dynamic LDS block is exactly lds_bytes (so ASan bounds it) and is refilled
+// with 0xff before every block, so that a read before a write within a block
+// shows up as NaN.
 inline void emu_launch(int64_t nblocks, int nthreads, size_t lds_bytes,
-                       const std::function<void()> &body) {
-  for (int64_t b = 0; b < nblocks; ++b) {
-    std::unique_ptr<unsigned char[]> lds(new unsigned char[std::max<size_t>(lds_bytes, 1)]);
-    std::memset(lds.get(), 0xff, lds_bytes);
-    emu_lds = lds.get();
-    std::barrier<> bar(nthreads);
-    emu_block_barrier = &bar;
-    std::vector<std::thread> th;
-    th.reserve(nthreads);
-    for (int t = 0; t < nthreads; ++t)
-      th.emplace_back([&, t] {
-        threadIdx.x = unsigned(t);
-        blockIdx.x = unsigned(b);
-        body();
-      });
-    for (auto &x : th) x.join();
-    emu_lds = nullptr;
+                       const std::function<void()> &body, int nblocks_y = 1) {
+  const int64_t total = nblocks * nblocks_y;
+  if (total <= 0) return;
+  std::unique_ptr<unsigned char[]> lds(new unsigned char[std::max<size_t>(lds_bytes, 1)]);
+  unsigned char *base = lds.get();
+  auto refill = [base, lds_bytes]() noexcept { std::memset(base, 0xff, lds_bytes); };
+  refill();
+  emu_lds = base;
+  std::barrier<> bar(nthreads);
+  emu_block_barrier = &bar;
+  std::barrier<decltype(refill)> next_block(nthreads, refill);
+  std::vector<std::unique_ptr<std::barrier<>>> waves;
+  for (int w = 0; w * 64 < nthreads; ++w) {
+    waves.emplace_back(new std::barrier<>(std::min(64, nthreads - 64 * w)));
+    emu_wave_barriers[w] = waves.back().get();
   }
+  std::vector<std::thread> th;
+  th.reserve(nthreads);
+  for (int t = 0; t < nthreads; ++t)
+    th.emplace_back([&, t] {
+      threadIdx.x = unsigned(t);
+      for (int64_t g = 0; g < total; ++g) {
+        blockIdx.x = unsigned(g % nblocks);
+        blockIdx.y = unsigned(g / nblocks);
+        body();
+        next_block.arrive_and_wait();
+      }
+    });
+  for (auto &x : th) x.join();
+  emu_lds = nullptr;
 }

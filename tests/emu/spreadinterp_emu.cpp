@@ -1,0 +1,176 @@
+// Runs the spreading / interpolation kernels of finufft_amd/csrc/spreadinterp.hip
+// (compiled from that source for the host, FA_REAL / FA_DIM / EMU_NS set on
+// the command line) on seeded points and writes inputs and outputs for the
+// test to check against a direct FP64 sum. The point sort and subproblem
+// table are built here with the library's own first-node rule (start_index).
+//   spreadinterp_emu <gm|sub|colored|bin|binsub|igm> nf0 nf1 nf2 b0 b1 b2 M maxsub
+//                    blk clustered seed out.bin
+// Exit codes: 0 ok, 2 usage, 3 geometry not valid for the variant.
+#include "hip_shim.hpp"
+#include "spreadinterp_kernels.inc"
+
+#include <random>
+#include <string>
+
+using namespace fa;
+using T = FA_REAL;
+constexpr int DIM = FA_DIM;
+constexpr int NS = EMU_NS;
+using C = cplx_t<T>;
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename X>
+X *alloc(size_t n) {
+  return static_cast<X *>(std::malloc(sizeof(X) * std::max<size_t>(n, 1)));
+}
+
+int main(int argc, char **argv) {
+  if (argc != 14) return 2;
+  const std::string var = argv[1];
+  int nf[3], bs[3];
+  for (int d = 0; d < 3; ++d) nf[d] = d < DIM ? std::atoi(argv[2 + d]) : 1;
+  for (int d = 0; d < 3; ++d) bs[d] = d < DIM ? std::atoi(argv[5 + d]) : 1;
+  const int M = std::atoi(argv[8]), maxsub = std::atoi(argv[9]), blk = std::atoi(argv[10]);
+  const int clustered = std::atoi(argv[11]), seed = std::atoi(argv[12]);
+  const char *out = argv[13];
+  const bool spreads = var == "gm" || var == "sub" || var == "colored";
+  if (var == "colored")  // whole-bin colors need an even number (>= 2) of equal bins
+    for (int d = 0; d < DIM; ++d) {
+      const int n = nf[d] / bs[d];
+      if (nf[d] % bs[d] || n < 2 || n % 2 || bs[d] < NS - 1) return 3;
+    }
+
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<double> U(-kPi, kPi), V(-1, 1);
+  T *x[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < DIM; ++d) {
+    x[d] = alloc<T>(M);
+    for (int i = 0; i < M; ++i)
+      x[d][i] = T(clustered && i % 2 ? -kPi + 0.3 * (U(rng) + kPi) / (2 * kPi) : U(rng));
+  }
+  // first-node bins, stable order (the device sort's rule)
+  int nb[3] = {1, 1, 1};
+  for (int d = 0; d < DIM; ++d) nb[d] = (nf[d] + bs[d] - 1) / bs[d];
+  const int nbt = nb[0] * nb[1] * nb[2];
+  std::vector<int> key(M), cnt(nbt + 1, 0);
+  const int fold64 = sizeof(T) == 4 ? 1 : 0;
+  for (int i = 0; i < M; ++i) {
+    int k = 0, stride = 1;
+    for (int d = 0; d < DIM; ++d) {
+      int s = start_index<T>(x[d][i], nf[d], NS, fold64);
+      if (s < 0) s += nf[d];
+      k += std::min(s / bs[d], nb[d] - 1) * stride;
+      stride *= nb[d];
+    }
+    key[i] = k;
+    ++cnt[k + 1];
+  }
+  for (int b = 0; b < nbt; ++b) cnt[b + 1] += cnt[b];
+  int *idx = alloc<int>(M), *binstart = alloc<int>(nbt + 1);
+  {
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < M; ++i) idx[pos[key[i]]++] = i;
+    std::copy(cnt.begin(), cnt.end(), binstart);
+  }
+  int *substart = alloc<int>(nbt + 1);
+  substart[0] = 0;
+  for (int b = 0; b < nbt; ++b)
+    substart[b + 1] = substart[b] + (binstart[b + 1] - binstart[b] + maxsub - 1) / maxsub;
+  const int nsub = substart[nbt];
+  const int cap = nbt + (M + maxsub - 1) / maxsub;  // the library's launch bound
+  int *sub2bin = alloc<int>(cap);
+  for (int b = 0; b < nbt; ++b)
+    for (int s = substart[b]; s < substart[b + 1]; ++s) sub2bin[s] = b;
+  T *sx[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < DIM; ++d) {
+    sx[d] = alloc<T>(M);
+    for (int i = 0; i < M; ++i) sx[d][i] = x[d][idx[i]];
+  }
+
+  SpreadCtx<T> ctx;
+  ctx.dim = DIM;
+  ctx.ns = NS;
+  ctx.method = var == "gm" || var == "igm" ? 1 : 2;
+  ctx.M = M;
+  for (int d = 0; d < 3; ++d) {
+    ctx.nf[d] = nf[d];
+    ctx.xyz[d] = x[d];
+    ctx.sxyz[d] = sx[d];
+    ctx.binsize[d] = bs[d];
+    ctx.nbins[d] = nb[d];
+  }
+  ctx.kp.beta = T(2.30 * NS);
+  ctx.kp.es_c = T(4.0 / (double(NS) * NS));
+  ctx.kp.horner = 0;  // direct exp(sqrt()) evaluation: no coefficient table
+  ctx.kp.fold64 = fold64;
+  ctx.kp.coef = nullptr;
+  ctx.idx = idx;
+  ctx.binstart = binstart;
+  ctx.maxsub = maxsub;
+  ctx.substart = substart;
+  ctx.sub2bin = sub2bin;
+  ctx.sub_capacity = cap;
+  int64_t ntile = 1;
+  for (int d = 0; d < DIM; ++d) ntile *= bs[d] + NS - 1;
+  ctx.lds_bytes = size_t(ntile) * 2 * sizeof(T);
+  ctx.lds_spread = size_t(ntile) * 2 * sizeof(double);
+  ctx.interp_sub = var == "binsub" ? 1 : 0;
+
+  const int64_t nftot = int64_t(nf[0]) * nf[1] * nf[2];
+  C *c = alloc<C>(size_t(M) * blk);
+  C *fw = alloc<C>(size_t(nftot) * blk);
+  if (spreads) {
+    for (int64_t i = 0; i < int64_t(M) * blk; ++i) c[i] = {T(V(rng)), T(V(rng))};
+    for (int64_t i = 0; i < nftot * blk; ++i) fw[i] = {T(0), T(0)};
+  } else {
+    for (int64_t i = 0; i < nftot * blk; ++i) fw[i] = {T(V(rng)), T(V(rng))};
+    for (int64_t i = 0; i < int64_t(M) * blk; ++i) c[i] = {T(NAN), T(NAN)};
+  }
+
+  if (var == "gm") {
+    const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
+    emu_launch(g, SPREAD_THREADS, 0, [&] { spread_gm_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
+  } else if (var == "sub") {
+    emu_launch(cap, BIN_THREADS, ctx.lds_spread,
+               [&] { spread_sub_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
+  } else if (var == "colored") {
+    ctx.colored = 1;
+    for (int color = 0; color < (1 << DIM); ++color) {
+      int col[3] = {0, 0, 0};
+      int64_t n = 1;
+      for (int d = 0; d < DIM; ++d) {
+        col[d] = (color >> d) & 1;
+        n *= (nb[d] - col[d] + 1) / 2;
+      }
+      emu_launch(n, BIN_THREADS, ctx.lds_spread, [&] {
+        spread_bin_colored_kernel<T, DIM, NS>(ctx, c, fw, nftot, col[0], col[1], col[2]);
+      }, blk);
+    }
+  } else if (var == "bin" || var == "binsub") {
+    emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
+               [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
+  } else if (var == "igm") {
+    const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
+    emu_launch(g, SPREAD_THREADS, 0, [&] { interp_gm_kernel<T, DIM, NS>(ctx, c, fw, nftot, blk); });
+  } else {
+    return 2;
+  }
+
+  FILE *f = std::fopen(out, "wb");
+  if (!f) return 2;
+  auto put = [&](double v) { std::fwrite(&v, sizeof(double), 1, f); };
+  put(double(nsub));
+  for (int d = 0; d < DIM; ++d)
+    for (int i = 0; i < M; ++i) put(double(x[d][i]));
+  for (int64_t i = 0; i < int64_t(M) * blk; ++i) put(double(c[i].x)), put(double(c[i].y));
+  for (int64_t i = 0; i < nftot * blk; ++i) put(double(fw[i].x)), put(double(fw[i].y));
+  std::fclose(f);
+  for (int d = 0; d < DIM; ++d) std::free(x[d]), std::free(sx[d]);
+  std::free(idx);
+  std::free(binstart);
+  std::free(substart);
+  std::free(sub2bin);
+  std::free(c);
+  std::free(fw);
+  return 0;
+}
