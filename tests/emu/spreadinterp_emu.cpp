@@ -3,7 +3,8 @@
 // the command line) on seeded points and writes inputs and outputs for the
 // test to check against a direct FP64 sum. The point sort and subproblem
 // table are built here with the library's own first-node rule (start_index).
-//   spreadinterp_emu <gm|sub|colored|bin|binsub|igm> nf0 nf1 nf2 b0 b1 b2 M maxsub
+//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|bin|binsub|igm>
+//                    nf0 nf1 nf2 b0 b1 b2 M maxsub
 //                    blk clustered seed out.bin
 // Exit codes: 0 ok, 2 usage, 3 geometry not valid for the variant.
 #include "hip_shim.hpp"
@@ -33,8 +34,12 @@ int main(int argc, char **argv) {
   const int M = std::atoi(argv[8]), maxsub = std::atoi(argv[9]), blk = std::atoi(argv[10]);
   const int clustered = std::atoi(argv[11]), seed = std::atoi(argv[12]);
   const char *out = argv[13];
-  const bool spreads = var == "gm" || var == "sub" || var == "colored";
-  if (var == "colored")  // whole-bin colors need an even number (>= 2) of equal bins
+  const bool col = var == "col" || var == "colsub" || var == "colmix";
+  const bool spreads = var == "gm" || var == "sub" || var == "colored" || col;
+  if (col && !(DIM == 3 && sizeof(T) == 4 && NS <= COL_MAX_NS && bs[0] == 8 && bs[1] == 8))
+    return 3;
+  if (var == "colored" || var == "col" || var == "colmix")
+    // whole-bin colors need an even number (>= 2) of equal bins
     for (int d = 0; d < DIM; ++d) {
       const int n = nf[d] / bs[d];
       if (nf[d] % bs[d] || n < 2 || n % 2 || bs[d] < NS - 1) return 3;
@@ -52,30 +57,47 @@ int main(int argc, char **argv) {
   int nb[3] = {1, 1, 1};
   for (int d = 0; d < DIM; ++d) nb[d] = (nf[d] + bs[d] - 1) / bs[d];
   const int nbt = nb[0] * nb[1] * nb[2];
-  std::vector<int> key(M), cnt(nbt + 1, 0);
+  // Column spreading refines the key inside a bin to (sub-column of S x S
+  // first nodes in x, y; local z of the first node), the library's sort key.
+  const int S = col ? col_side(NS) : 1, NSX = col ? 8 / S : 1, NSUB = NSX * NSX;
+  const int nz = col ? bs[2] : 1;
+  const int nkeys = nbt * NSUB * nz;
+  std::vector<int> key(M), kcnt(nkeys + 1, 0);
   const int fold64 = sizeof(T) == 4 ? 1 : 0;
   for (int i = 0; i < M; ++i) {
-    int k = 0, stride = 1;
+    int k = 0, stride = 1, loc[3] = {0, 0, 0};
     for (int d = 0; d < DIM; ++d) {
       int s = start_index<T>(x[d][i], nf[d], NS, fold64);
       if (s < 0) s += nf[d];
-      k += std::min(s / bs[d], nb[d] - 1) * stride;
+      const int b = std::min(s / bs[d], nb[d] - 1);
+      loc[d] = s - b * bs[d];
+      k += b * stride;
       stride *= nb[d];
     }
+    if (col) k = (k * NSUB + (loc[0] / S) + NSX * (loc[1] / S)) * nz + loc[2];
     key[i] = k;
-    ++cnt[k + 1];
+    ++kcnt[k + 1];
   }
-  for (int b = 0; b < nbt; ++b) cnt[b + 1] += cnt[b];
+  for (int b = 0; b < nkeys; ++b) kcnt[b + 1] += kcnt[b];
+  std::vector<int> cnt(nbt + 1);
+  for (int b = 0; b <= nbt; ++b) cnt[b] = kcnt[int64_t(b) * NSUB * nz];
   int *idx = alloc<int>(M), *binstart = alloc<int>(nbt + 1);
+  int *colstart = alloc<int>(size_t(nbt) * NSUB + 1);
   {
-    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    std::vector<int> pos(kcnt.begin(), kcnt.end() - 1);
     for (int i = 0; i < M; ++i) idx[pos[key[i]]++] = i;
     std::copy(cnt.begin(), cnt.end(), binstart);
+    for (int q = 0; q <= nbt * NSUB; ++q) colstart[q] = kcnt[int64_t(q) * nz];
   }
+  // Column spreading: colored launches take the bins of at most dense_min
+  // points; the subproblem table covers the denser bins only.
+  const int dense_min = var == "col" ? M : var == "colmix" ? 8 * maxsub : 0;
   int *substart = alloc<int>(nbt + 1);
   substart[0] = 0;
-  for (int b = 0; b < nbt; ++b)
-    substart[b + 1] = substart[b] + (binstart[b + 1] - binstart[b] + maxsub - 1) / maxsub;
+  for (int b = 0; b < nbt; ++b) {
+    const int n = binstart[b + 1] - binstart[b];
+    substart[b + 1] = substart[b] + (n > dense_min ? (n + maxsub - 1) / maxsub : 0);
+  }
   const int nsub = substart[nbt];
   const int cap = nbt + (M + maxsub - 1) / maxsub;  // the library's launch bound
   int *sub2bin = alloc<int>(cap);
@@ -115,6 +137,13 @@ int main(int argc, char **argv) {
   ctx.lds_bytes = size_t(ntile) * 2 * sizeof(T);
   ctx.lds_spread = size_t(ntile) * 2 * sizeof(double);
   ctx.interp_sub = var == "binsub" ? 1 : 0;
+  if (col) {
+    ctx.colS = S;
+    ctx.colstart = colstart;
+    ctx.dense_min = dense_min;
+    ctx.n_dense_sub = nsub;
+    ctx.lds_spread = size_t(ntile) * 16 + col_stage_bytes();
+  }
 
   const int64_t nftot = int64_t(nf[0]) * nf[1] * nf[2];
   C *c = alloc<C>(size_t(M) * blk);
@@ -146,6 +175,21 @@ int main(int argc, char **argv) {
         spread_bin_colored_kernel<T, DIM, NS>(ctx, c, fw, nftot, col[0], col[1], col[2]);
       }, blk);
     }
+  } else if (col) {
+    if constexpr (DIM == 3 && sizeof(T) == 4 && NS <= COL_MAX_NS) {
+      if (var != "colsub")
+        for (int color = 0; color < 8; ++color) {
+          const int col3[3] = {color & 1, (color >> 1) & 1, (color >> 2) & 1};
+          int64_t n = 1;
+          for (int d = 0; d < 3; ++d) n *= (nb[d] - col3[d] + 1) / 2;
+          emu_launch(n, 64 * COL_WAVES, ctx.lds_spread, [&] {
+            spread_col_colored_kernel<NS>(ctx, c, fw, nftot, col3[0], col3[1], col3[2]);
+          }, blk);
+        }
+      if (nsub > 0)
+        emu_launch(nsub, 64 * COL_WAVES, ctx.lds_spread,
+                   [&] { spread_col_sub_kernel<NS>(ctx, c, fw, nftot); }, blk);
+    }
   } else if (var == "bin" || var == "binsub") {
     emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
                [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
@@ -170,6 +214,7 @@ int main(int argc, char **argv) {
   std::free(binstart);
   std::free(substart);
   std::free(sub2bin);
+  std::free(colstart);
   std::free(c);
   std::free(fw);
   return 0;

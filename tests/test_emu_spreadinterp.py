@@ -7,7 +7,8 @@ This is the reference's spread-only / interp-only known-answer check
 (test/cuda/cufinufft1dspreadinterponly_test.cu:68-152) restated for the
 kernel source itself: point-driven spreading (method 1), LDS subproblem
 spreading with an atomic flush, whole-bin parity-colored spreading with a
-plain flush, LDS interpolation by bins and by subproblems, and point-driven
+plain flush, the 3-D column-window spreader (colored bins, dense-bin
+subproblems and their mix), LDS interpolation by bins and by subproblems, and point-driven
 interpolation, in 1-3 dimensions and both precisions, with uniform and
 clustered points, bins that do not divide the grid, tiles that wrap twice,
 and several vectors per launch. ASan checks every global and LDS access.
@@ -168,6 +169,37 @@ def test_interp_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
         err = np.linalg.norm(c[t] - ref) / np.linalg.norm(ref)
         print(var, geom, f"relerr {err:.2e}")
         assert err < (2e-6 if real == "float" else 1e-13)
+
+
+# fp32 3-D column-window spreader: (ns, nf, bins) with every sub-column side
+# S = 4, 2, 1 (ns <= 5, <= 7, 8), bins of 8 x 8 first nodes in x, y.
+COL_GEOMS = [
+    (4, (32, 32, 24), (8, 8, 6)),
+    (7, (32, 32, 24), (8, 8, 6)),
+    (8, (32, 16, 32), (8, 8, 8)),
+]
+
+
+@pytest.mark.parametrize("geom", COL_GEOMS, ids=lambda g: f"ns{g[0]}")
+@pytest.mark.parametrize("var,maxsub,clustered", [("col", 1024, 0), ("colsub", 100, 1),
+                                                  ("colmix", 8, 1)])
+def test_column_spreader_vs_direct(tmp_path_factory, tmp_path, geom, var, maxsub, clustered):
+    """Whole bins in parity colors (col), every bin as subproblems (colsub),
+    and the mix the library runs on clustered points: bins above 8 maxsub
+    points as subproblems, the rest colored (colmix)."""
+    ns, nf, bs = geom
+    exe = _exe(tmp_path_factory, "float", 3, ns)
+    blk = 2 if var == "col" else 1
+    nsub, xs, c, fw = _run(exe, tmp_path, var, nf, bs, 2000, maxsub, blk, clustered, 14)
+    if var == "colmix":
+        assert nsub > 0  # the cluster's bins went to the subproblem kernel
+    idx, w = _weights(xs, nf, ns, 2.30 * ns)
+    for t in range(blk):
+        ref = np.zeros(int(np.prod(nf)), complex)
+        np.add.at(ref, idx.ravel(), (w * c[t][:, None]).ravel())
+        err = np.linalg.norm(fw[t] - ref) / np.linalg.norm(ref)
+        print(var, geom, f"dense subproblems {nsub} relerr {err:.2e}")
+        assert err < 2e-6
 
 
 def test_bins_wider_than_half_the_grid(tmp_path_factory, tmp_path):
