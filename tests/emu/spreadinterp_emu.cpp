@@ -12,6 +12,7 @@
 
 #include <random>
 #include <string>
+#include <type_traits>
 
 using namespace fa;
 using T = FA_REAL;
@@ -176,20 +177,26 @@ int main(int argc, char **argv) {
       }, blk);
     }
   } else if (col) {
-    if constexpr (DIM == 3 && sizeof(T) == 4 && NS <= COL_MAX_NS) {
-      if (var != "colsub")
-        for (int color = 0; color < 8; ++color) {
-          const int col3[3] = {color & 1, (color >> 1) & 1, (color >> 2) & 1};
-          int64_t n = 1;
-          for (int d = 0; d < 3; ++d) n *= (nb[d] - col3[d] + 1) / 2;
-          emu_launch(n, 64 * COL_WAVES, ctx.lds_spread, [&] {
-            spread_col_colored_kernel<NS>(ctx, c, fw, nftot, col3[0], col3[1], col3[2]);
-          }, blk);
-        }
-      if (nsub > 0)
-        emu_launch(nsub, 64 * COL_WAVES, ctx.lds_spread,
-                   [&] { spread_col_sub_kernel<NS>(ctx, c, fw, nftot); }, blk);
-    }
+    // a generic lambda, so that the fp32 3-D kernels are only instantiated
+    // in the build they exist for
+    auto launch_col = [&](auto &cx, auto *cc, auto *ff) {
+      using TT = std::remove_cvref_t<decltype(cx.kp.beta)>;
+      if constexpr (DIM == 3 && std::is_same_v<TT, float> && NS <= COL_MAX_NS) {
+        if (var != "colsub")
+          for (int color = 0; color < 8; ++color) {
+            const int col3[3] = {color & 1, (color >> 1) & 1, (color >> 2) & 1};
+            int64_t n = 1;
+            for (int d = 0; d < 3; ++d) n *= (nb[d] - col3[d] + 1) / 2;
+            emu_launch(n, 64 * COL_WAVES, cx.lds_spread, [&] {
+              spread_col_colored_kernel<NS>(cx, cc, ff, nftot, col3[0], col3[1], col3[2]);
+            }, blk);
+          }
+        if (nsub > 0)
+          emu_launch(nsub, 64 * COL_WAVES, cx.lds_spread,
+                     [&] { spread_col_sub_kernel<NS>(cx, cc, ff, nftot); }, blk);
+      }
+    };
+    launch_col(ctx, c, fw);
   } else if (var == "bin" || var == "binsub") {
     emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
                [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
