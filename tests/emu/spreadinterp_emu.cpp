@@ -3,10 +3,11 @@
 // the command line) on seeded points and writes inputs and outputs for the
 // test to check against a direct FP64 sum. The point sort and subproblem
 // table are built here with the library's own first-node rule (start_index).
-//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|bin|binsub|igm>
+//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|bin|binsub|mv|igm>
 //                    nf0 nf1 nf2 b0 b1 b2 M maxsub
 //                    blk clustered seed out.bin
-// Exit codes: 0 ok, 2 usage, 3 geometry not valid for the variant.
+// Exit codes: 0 ok, 2 usage, 3 geometry not valid for the variant, 4 the two
+// problems of the multi-vector launch differ.
 #include "hip_shim.hpp"
 #include "spreadinterp_kernels.inc"
 
@@ -200,6 +201,36 @@ int main(int argc, char **argv) {
   } else if (var == "bin" || var == "binsub") {
     emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
                [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
+  } else if (var == "mv") {
+    // the batch path's multi-vector interpolation: two problems (the same
+    // points and grids, separate outputs) in one launch, all blk vectors per
+    // work unit; the second problem's values must equal the first's
+    ctx.interp_sub = 1;
+    InterpItem<T> *items = alloc<InterpItem<T>>(2);
+    const C **fwb = alloc<const C *>(2);
+    C **cb = alloc<C *>(2);
+    C *c2 = alloc<C>(size_t(M) * blk);
+    for (int64_t i = 0; i < int64_t(M) * blk; ++i) c2[i] = {T(NAN), T(NAN)};
+    for (int q = 0; q < 2; ++q) {
+      items[q] = InterpItem<T>{};
+      items[q].ctx = ctx;
+      items[q].nftot = nftot;
+      items[q].nsub = nsub;
+      items[q].unit_begin = int64_t(q) * nsub;
+      fwb[q] = fw;
+    }
+    cb[0] = c;
+    cb[1] = c2;
+    const int64_t units = 2 * int64_t(nsub);
+    emu_launch(units + 2, MV_THREADS, ctx.lds_bytes, [&] {
+      interp_batch_kernel<T, DIM, NS>(items, 2, ctx.kp, fwb, cb, 0, blk, units);
+    });
+    const bool same = std::memcmp(c, c2, sizeof(C) * size_t(M) * blk) == 0;
+    std::free(items);
+    std::free(fwb);
+    std::free(cb);
+    std::free(c2);
+    if (!same) return 4;
   } else if (var == "igm") {
     const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
     emu_launch(g, SPREAD_THREADS, 0, [&] { interp_gm_kernel<T, DIM, NS>(ctx, c, fw, nftot, blk); });

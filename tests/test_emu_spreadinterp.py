@@ -8,11 +8,13 @@ This is the reference's spread-only / interp-only known-answer check
 kernel source itself: point-driven spreading (method 1), LDS subproblem
 spreading with an atomic flush, whole-bin parity-colored spreading with a
 plain flush, the 3-D column-window spreader (colored bins, dense-bin
-subproblems and their mix), LDS interpolation by bins and by subproblems, and point-driven
+subproblems and their mix), LDS interpolation by bins and by subproblems,
+the batch path's multi-vector interpolation, and point-driven
 interpolation, in 1-3 dimensions and both precisions, with uniform and
 clustered points, bins that do not divide the grid, tiles that wrap twice,
 and several vectors per launch. ASan checks every global and LDS access.
-The GPU tests check the same kernels on the device (tests/test_gpu_parity.py).
+The GPU tests check the same kernels on the device (tests/test_gpu_parity.py,
+tests/test_gpu_batch.py).
 """
 
 import shutil
@@ -158,7 +160,7 @@ def test_spread_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
 
 
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
-@pytest.mark.parametrize("var", ["igm", "bin", "binsub"])
+@pytest.mark.parametrize("var", ["igm", "bin", "binsub", "mv"])
 def test_interp_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
     real, dim, ns, nf, bs, M, maxsub, blk, clustered = geom
     exe = _exe(tmp_path_factory, real, dim, ns)
