@@ -30,7 +30,13 @@
 struct emu_dim3 {
   unsigned x = 0, y = 0, z = 0;
 };
-inline thread_local emu_dim3 threadIdx, blockIdx;
+struct dim3 {
+  unsigned x = 1, y = 1, z = 1;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+inline thread_local emu_dim3 threadIdx, blockIdx, blockDim, gridDim;
+// static shared arrays of a kernel: one copy, used by one block at a time
+#define __shared__ static
 inline std::barrier<> *emu_block_barrier = nullptr;
 inline std::barrier<> *emu_wave_barriers[16] = {};
 inline unsigned char *emu_lds = nullptr;
@@ -130,6 +136,9 @@ inline void emu_launch(int64_t nblocks, int nthreads, size_t lds_bytes,
   for (int t = 0; t < nthreads; ++t)
     th.emplace_back([&, t] {
       threadIdx.x = unsigned(t);
+      blockDim.x = unsigned(nthreads);
+      gridDim.x = unsigned(nblocks);
+      gridDim.y = unsigned(nblocks_y);
       for (int64_t g = 0; g < total; ++g) {
         blockIdx.x = unsigned(g % nblocks);
         blockIdx.y = unsigned(g / nblocks);
