@@ -3,7 +3,7 @@
 // the command line) on seeded points and writes inputs and outputs for the
 // test to check against a direct FP64 sum. The point sort and subproblem
 // table are built here with the library's own first-node rule (start_index).
-//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|bin|binsub|mv|igm>
+//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|bin|binsub|mv|igm|grad>
 //                    nf0 nf1 nf2 b0 b1 b2 M maxsub
 //                    blk clustered seed out.bin
 // Exit codes: 0 ok, 2 usage, 3 geometry not valid for the variant, 4 the two
@@ -149,6 +149,7 @@ int main(int argc, char **argv) {
 
   const int64_t nftot = int64_t(nf[0]) * nf[1] * nf[2];
   C *c = alloc<C>(size_t(M) * blk);
+  C *dc = nullptr;
   C *fw = alloc<C>(size_t(nftot) * blk);
   if (spreads) {
     for (int64_t i = 0; i < int64_t(M) * blk; ++i) c[i] = {T(V(rng)), T(V(rng))};
@@ -231,6 +232,13 @@ int main(int argc, char **argv) {
     std::free(cb);
     std::free(c2);
     if (!same) return 4;
+  } else if (var == "grad") {
+    // values and coordinate derivatives of the interpolation (autograd)
+    dc = alloc<C>(size_t(DIM) * M * blk);
+    for (int64_t i = 0; i < int64_t(DIM) * M * blk; ++i) dc[i] = {T(NAN), T(NAN)};
+    const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
+    emu_launch(g, SPREAD_THREADS, 0,
+               [&] { interp_grad_kernel<T, DIM, NS>(ctx, c, dc, fw, nftot, blk); });
   } else if (var == "igm") {
     const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
     emu_launch(g, SPREAD_THREADS, 0, [&] { interp_gm_kernel<T, DIM, NS>(ctx, c, fw, nftot, blk); });
@@ -246,7 +254,10 @@ int main(int argc, char **argv) {
     for (int i = 0; i < M; ++i) put(double(x[d][i]));
   for (int64_t i = 0; i < int64_t(M) * blk; ++i) put(double(c[i].x)), put(double(c[i].y));
   for (int64_t i = 0; i < nftot * blk; ++i) put(double(fw[i].x)), put(double(fw[i].y));
+  if (dc)
+    for (int64_t i = 0; i < int64_t(DIM) * M * blk; ++i) put(double(dc[i].x)), put(double(dc[i].y));
   std::fclose(f);
+  std::free(dc);
   for (int d = 0; d < DIM; ++d) std::free(x[d]), std::free(sx[d]);
   std::free(idx);
   std::free(binstart);

@@ -9,8 +9,8 @@ kernel source itself: point-driven spreading (method 1), LDS subproblem
 spreading with an atomic flush, whole-bin parity-colored spreading with a
 plain flush, the 3-D column-window spreader (colored bins, dense-bin
 subproblems and their mix), LDS interpolation by bins and by subproblems,
-the batch path's multi-vector interpolation, and point-driven
-interpolation, in 1-3 dimensions and both precisions, with uniform and
+the batch path's multi-vector interpolation, point-driven interpolation
+and its coordinate derivatives (the autograd kernel), in 1-3 dimensions and both precisions, with uniform and
 clustered points, bins that do not divide the grid, tiles that wrap twice,
 and several vectors per launch. ASan checks every global and LDS access.
 The GPU tests check the same kernels on the device (tests/test_gpu_parity.py,
@@ -202,6 +202,70 @@ def test_column_spreader_vs_direct(tmp_path_factory, tmp_path, geom, var, maxsub
         err = np.linalg.norm(fw[t] - ref) / np.linalg.norm(ref)
         print(var, geom, f"dense subproblems {nsub} relerr {err:.2e}")
         assert err < 2e-6
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
+def test_interp_grad_kernel_vs_direct(tmp_path_factory, tmp_path, geom):
+    """Values and coordinate derivatives of the interpolation (the coordinate
+    part of the autograd backward): dc/dx_d = sum fw * d/dx_d prod phi, with
+    d phi(s + a - r)/dx = -phi'(s + a - r) nf / (2 pi)."""
+    real, dim, ns, nf, bs, M, maxsub, blk, clustered = geom
+    exe = _exe(tmp_path_factory, real, dim, ns)
+    out = tmp_path / "grad.bin"
+    nf3 = list(nf) + [1] * (3 - dim)
+    bs3 = list(bs) + [1] * (3 - dim)
+    res = subprocess.run([str(exe), "grad", *map(str, nf3), *map(str, bs3), str(M), str(maxsub),
+                          str(blk), str(clustered), "15", str(out)],
+                         capture_output=True, text=True, timeout=900)
+    assert res.returncode == 0, (res.returncode, res.stderr[-4000:])
+    v = np.fromfile(out, dtype=np.float64)
+    p = 1
+    xs = [v[p + d * M:p + (d + 1) * M] for d in range(dim)]
+    p += dim * M
+    c = v[p:p + 2 * M * blk]
+    p += 2 * M * blk
+    ng = int(np.prod(nf))
+    g = v[p:p + 2 * ng * blk]
+    p += 2 * ng * blk
+    dcv = v[p:]
+    c = (c[0::2] + 1j * c[1::2]).reshape(blk, M)
+    fw = (g[0::2] + 1j * g[1::2]).reshape(blk, ng)
+    dc = (dcv[0::2] + 1j * dcv[1::2]).reshape(dim, blk, M)
+    beta, cc = 2.30 * ns, 4.0 / (ns * ns)
+    per = []
+    for d in range(dim):
+        r = np.mod(xs[d] / (2 * np.pi) + 0.5, 1.0) * nf[d]
+        s = np.ceil(r - ns / 2.0)
+        a = s[:, None] + np.arange(ns)[None, :] - r[:, None]
+        z = np.maximum(1 - cc * a * a, 0)
+        phi = np.where(cc * a * a < 1, np.exp(beta * (np.sqrt(z) - 1)), 0.0)
+        dphi = np.where(cc * a * a < 1, -phi * beta * cc * a / np.sqrt(np.where(z > 0, z, 1)), 0.0)
+        dx = -dphi * nf[d] / (2 * np.pi)
+        cells = np.mod(s[:, None].astype(np.int64) + np.arange(ns)[None, :], nf[d])
+        per.append((cells, phi, dx))
+
+    def outer(which):
+        idx = np.zeros((M, 1), np.int64)
+        w = np.ones((M, 1))
+        stride = 1
+        for d in range(dim):
+            cells, phi, dx = per[d]
+            k = dx if d == which else phi
+            idx = (idx[:, :, None] + stride * cells[:, None, :]).reshape(M, -1)
+            w = (w[:, :, None] * k[:, None, :]).reshape(M, -1)
+            stride *= nf[d]
+        return idx, w
+
+    tol = 2e-6 if real == "float" else 1e-12
+    for t in range(blk):
+        idx, w = outer(-1)
+        ref = (fw[t][idx] * w).sum(axis=1)
+        assert np.linalg.norm(c[t] - ref) / np.linalg.norm(ref) < tol
+        for d in range(dim):
+            idx, w = outer(d)
+            ref = (fw[t][idx] * w).sum(axis=1)
+            err = np.linalg.norm(dc[d, t] - ref) / np.linalg.norm(ref)
+            assert err < tol, (d, err)
 
 
 def test_bins_wider_than_half_the_grid(tmp_path_factory, tmp_path):
