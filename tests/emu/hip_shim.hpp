@@ -45,7 +45,9 @@ inline void __syncthreads() { emu_block_barrier->arrive_and_wait(); }
 // A wave's lanes run as separate threads: the points where the device code
 // relies on the wave executing together (wave barriers, cross-lane reads)
 // become barriers over the wave's 64 threads. Fences have no work left to do.
-inline void __builtin_amdgcn_wave_barrier() { emu_wave_barriers[threadIdx.x >> 6]->arrive_and_wait(); }
+inline void __builtin_amdgcn_wave_barrier() {
+  emu_wave_barriers[threadIdx.x >> 6]->arrive_and_wait();
+}
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 inline uint64_t emu_lane_slots[16][64];
 template <typename V>

@@ -10,9 +10,9 @@ spreading with an atomic flush, whole-bin parity-colored spreading with a
 plain flush, the 3-D column-window spreader (colored bins, dense-bin
 subproblems and their mix), LDS interpolation by bins and by subproblems,
 the batch path's multi-vector interpolation, point-driven interpolation
-and its coordinate derivatives (the autograd kernel), in 1-3 dimensions and both precisions, with uniform and
-clustered points, bins that do not divide the grid, tiles that wrap twice,
-and several vectors per launch. ASan checks every global and LDS access.
+and its coordinate derivatives (the autograd kernel), in 1-3 dimensions and
+both precisions, with uniform and clustered points, bins that do not divide
+the grid, tiles that wrap twice, and several vectors per launch. ASan checks every global and LDS access.
 The GPU tests check the same kernels on the device (tests/test_gpu_parity.py,
 tests/test_gpu_batch.py).
 """
