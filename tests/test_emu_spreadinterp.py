@@ -268,6 +268,20 @@ def test_interp_grad_kernel_vs_direct(tmp_path_factory, tmp_path, geom):
             assert err < tol, (d, err)
 
 
+def test_column_spreader_spread_only_grid(tmp_path_factory, tmp_path):
+    """The geometry of a spread-only 3-D fp32 plan on a 48 x 40 x 44 mode grid
+    (nf = the mode counts, x fastest: 44 x 40 x 48): bins of 8 x 8 x 10 whose
+    x count does not divide the grid, so every bin runs as subproblems of the
+    column spreader, on clustered points."""
+    exe = _exe(tmp_path_factory, "float", 3, 7)
+    nf, bs = (44, 40, 48), (8, 8, 10)
+    nsub, xs, c, fw = _run(exe, tmp_path, "colsub", nf, bs, 4000, 1024, 1, 1, 16)
+    idx, w = _weights(xs, nf, 7, 2.30 * 7)
+    ref = np.zeros(int(np.prod(nf)), complex)
+    np.add.at(ref, idx.ravel(), (w * c[0][:, None]).ravel())
+    assert np.linalg.norm(fw[0] - ref) / np.linalg.norm(ref) < 2e-6
+
+
 def test_bins_wider_than_half_the_grid(tmp_path_factory, tmp_path):
     """Bins of 16 on a grid of 18: the padded tile reaches past 2 nf, and the
     interpolation stages every tile cell (zero padding included), whose grid
