@@ -66,6 +66,40 @@ inline V __builtin_amdgcn_readlane(V v, int k) {
   return r;
 }
 
+// Lane index in the wave, and the up-shuffle of a wave (lanes below the
+// offset keep their own value), as a collective over the wave's threads.
+inline unsigned __lane_id() { return threadIdx.x & 63; }
+template <typename V>
+inline V __shfl_up(V v, unsigned o, int width = 64) {
+  static_assert(sizeof(V) <= 8, "shuffle of a 32- or 64-bit value");
+  (void)width;
+  const unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t bits = 0;
+  std::memcpy(&bits, &v, sizeof(V));
+  emu_wave_barriers[w]->arrive_and_wait();
+  emu_lane_slots[w][lane] = bits;
+  emu_wave_barriers[w]->arrive_and_wait();
+  if (lane >= o) bits = emu_lane_slots[w][lane - o];
+  emu_wave_barriers[w]->arrive_and_wait();
+  V r;
+  std::memcpy(&r, &bits, sizeof(V));
+  return r;
+}
+
+// Ballot of a predicate over the wave (a collective), and the popcount.
+inline unsigned long long __ballot(int pred) {
+  const unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  emu_wave_barriers[w]->arrive_and_wait();
+  emu_lane_slots[w][lane] = pred ? 1 : 0;
+  emu_wave_barriers[w]->arrive_and_wait();
+  unsigned long long m = 0;
+  for (int k = 0; k < 64; ++k)
+    if (emu_lane_slots[w][k]) m |= 1ull << k;
+  emu_wave_barriers[w]->arrive_and_wait();
+  return m;
+}
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+
 // Atomic adds (LDS or global) as host atomics.
 template <typename V>
 inline V atomicAdd(V *p, V v) {
