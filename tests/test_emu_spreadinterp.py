@@ -12,9 +12,9 @@ subproblems and their mix), LDS interpolation by bins and by subproblems,
 the batch path's multi-vector interpolation, point-driven interpolation
 and its coordinate derivatives (the autograd kernel), in 1-3 dimensions and
 both precisions, with uniform and clustered points, bins that do not divide
-the grid, tiles that wrap twice, and several vectors per launch. ASan checks every global and LDS access.
-The GPU tests check the same kernels on the device (tests/test_gpu_parity.py,
-tests/test_gpu_batch.py).
+the grid, tiles that wrap twice, and several vectors per launch. ASan checks
+every global and LDS access. The GPU tests check the same kernels on the
+device (tests/test_gpu_parity.py, tests/test_gpu_batch.py).
 """
 
 import shutil
