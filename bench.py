@@ -245,6 +245,41 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
     }
 
 
+PROBE_MODES = ("ds_add_f32", "ds_add_f64", "ds_add_u32", "rmw_f32", "ds_read_b32",
+               "ds_read_b32_uniform", "ds_read_b128", "ds_read_b128_uniform", "v_fma_f32")
+
+
+def lds_probe():
+    """Wave-level operations per clock per CU of the LDS operations the
+    spreading kernels choose between (and of v_fma_f32 for scale)."""
+    import ctypes
+
+    import torch
+
+    from finufft_amd import _lib
+
+    assert torch.cuda.is_available(), "the probe needs a GPU"
+    props = torch.cuda.get_device_properties(0)
+    ncu = props.multi_processor_count
+    clk_khz = ctypes.c_int(0)
+    hip = ctypes.CDLL("libamdhip64.so")
+    # hipDeviceAttributeClockRate (kHz)
+    hip.hipDeviceGetAttribute(ctypes.byref(clk_khz), 5, 0)
+    clk = clk_khz.value * 1e3 if clk_khz.value > 0 else 2.4e9
+    lib = _lib.lib()
+    out = {"cus": ncu, "clock_hz": clk, "rates": {}}
+    for mode, name in enumerate(PROBE_MODES):
+        ms, ops = ctypes.c_double(0), ctypes.c_double(0)
+        ier = lib.finufft_amd_lds_probe(mode, 8 * ncu, 2001, ctypes.byref(ms), ctypes.byref(ops))
+        if ier != 0:
+            raise RuntimeError(f"lds probe {name}: error {ier}")
+        rate = ops.value / (ms.value * 1e-3) / clk / ncu
+        out["rates"][name] = round(rate, 4)
+        log(f"lds probe {name}: {ms.value:.3f} ms, {rate:.3f} wave ops/clk/CU")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,7 +304,13 @@ def main(argv=None):
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a tensor of this many MiB (a plain copy kernel of "
                          "known size, the reference point for HBM counter readings)")
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="only measure LDS / VALU operation rates (wave instructions per clock "
+                         "per CU) with the library's probe kernels and print them as JSON")
     args = ap.parse_args(argv)
+
+    if args.lds_probe:
+        return lds_probe()
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "

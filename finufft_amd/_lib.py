@@ -99,6 +99,9 @@ def _declare(lib):
     lib.finufft_amd_set_stage_timing.restype = c_int
     lib.finufft_amd_get_stage_times.argtypes = [vp, c_int, POINTER(c_double)]
     lib.finufft_amd_get_stage_times.restype = c_int
+    lib.finufft_amd_lds_probe.argtypes = [c_int, c_int, c_int, POINTER(c_double),
+                                          POINTER(c_double)]
+    lib.finufft_amd_lds_probe.restype = c_int
     lib.finufft_amd_next235.argtypes = [c_int64, c_int64]
     lib.finufft_amd_next235.restype = c_int64
     lib.finufft_amd_gaussquad.argtypes = [c_int, POINTER(c_double), POINTER(c_double)]

@@ -229,10 +229,17 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
   hip_check(hipStreamCreateWithFlags(&own_, hipStreamNonBlocking), "hipStreamCreate(batch)");
   hip_check(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming), "hipEventCreate(batch)");
   hip_check(hipEventCreateWithFlags(&ev_out_, hipEventDisableTiming), "hipEventCreate(batch)");
+  // The batch copied every plan's sort tables and subproblem counts: it is
+  // valid for the points set so far, and the plans must outlive it.
+  for (Plan<T> *p : plans_) {
+    gens_.push_back(p->setpts_generation());
+    p->add_batch_ref(1);
+  }
 }
 
 template <typename T>
 BatchPlan<T>::~BatchPlan() {
+  for (size_t i = 0; i < gens_.size(); ++i) plans_[i]->add_batch_ref(-1);
   if (own_) (void)hipStreamSynchronize(own_);
   if (gexec_) (void)hipGraphExecDestroy(gexec_);
   if (graph_) (void)hipGraphDestroy(graph_);
@@ -268,6 +275,11 @@ template <typename T>
 void BatchPlan<T>::execute(C *const *c, C *const *fk) {
   device_guard dg(plans_[0]->opts_.gpu_device_id);
   const int n = nprob();
+  for (int i = 0; i < n; ++i)
+    if (plans_[i]->setpts_generation() != gens_[i])
+      fail(FINUFFT_ERR_INVALID_ARGUMENT,
+           "batch: the points of plan " + std::to_string(i) +
+               " were set again after the batch was made; make the batch again");
   for (int i = 0; i < n; ++i)
     if (c[i] == nullptr || fk[i] == nullptr)
       fail(FINUFFT_ERR_INVALID_ARGUMENT, "batch: null data pointer");

@@ -78,6 +78,7 @@ class BatchPlan {
   void enqueue(hipStream_t s);
 
   std::vector<Plan<T> *> plans_;
+  std::vector<uint64_t> gens_;  // each plan's setpts generation when the batch was made
   hipStream_t stream_ = nullptr;  // the plans' (caller's) stream
   // The schedule runs on a stream of its own (the legacy null stream cannot
   // be captured), ordered after and before the caller's stream by events.

@@ -21,6 +21,10 @@
 #include "mlp.hpp"
 #include "plan.hpp"
 
+namespace fa {
+void lds_probe(int mode, int blocks, int reps, double *ms, double *wave_ops);
+}
+
 using fa::Plan;
 using fa::safe_call;
 
@@ -83,9 +87,14 @@ Plan<T> *as_plan(void *p) {
 template <typename T>
 void destroy_plan(void *p) {
   if (!p) fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "null plan");
+  Plan<T> *plan = as_plan<T>(p);
+  // A batch holds raw pointers to its plans: a plan is destroyed only after
+  // every batch made from it.
+  if (plan->batch_refs() > 0)
+    fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "plan is held by a live batch; destroy the batch first");
   if (!Live<Plan<T>>::get().remove(p))
     fa::fail(FINUFFT_ERR_PLAN_NOTVALID, "not a live plan of this precision");
-  delete static_cast<Plan<T> *>(p);
+  delete plan;
 }
 
 template <typename T, typename P>
@@ -399,6 +408,13 @@ int finufft_amd_get_stage_times(void *plan, int is_double, double *ms) {
       as_plan<double>(plan)->timer().collect(ms);
     else
       as_plan<float>(plan)->timer().collect(ms);
+  });
+}
+
+int finufft_amd_lds_probe(int mode, int blocks, int reps, double *ms, double *wave_ops) {
+  return safe_call([&] {
+    if (!ms || !wave_ops) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "null output");
+    fa::lds_probe(mode, blocks, reps, ms, wave_ops);
   });
 }
 

@@ -174,6 +174,7 @@ bool choose_col_bins(const int *nf, int ns, size_t budget, size_t limit, int *ou
     const bool want_col = pass < 2;
     const int64_t cap = int64_t(pass % 2 == 0 ? budget : limit);
     for (int bz : zc) {
+      if (bz > nf[2]) continue;  // a bin never exceeds its grid
       const int b[3] = {8, 8, bz};
       if (tile_cells(3, b, ns) * 16 + int64_t(col_stage_bytes()) > cap) continue;
       const bool colr = colorable(3, nf, b, ns);
@@ -242,6 +243,11 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
   if (ntransf_ < 1) fail(FINUFFT_ERR_NTRANS_NOTVALID, "invalid ntransf");
 
   opts_.gpu_maxbatchsize = std::max(opts_.gpu_maxbatchsize, 0);
+  // The caller's bin sizes; the sizes the plan settles on live in geom_ only,
+  // so that a later type-3 setpts chooses afresh for its new grid.
+  user_bin_[0] = opts_.gpu_binsizex;
+  user_bin_[1] = opts_.gpu_binsizey;
+  user_bin_[2] = opts_.gpu_binsizez;
   if (opts_.gpu_method != 3) opts_.gpu_np = 0;
   if (type_ != 3) {
     for (int d = 0; d < 3; ++d) ms_[d] = d < dim_ ? nmodes[d] : 1;
@@ -260,8 +266,8 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
     fail(FINUFFT_ERR_INSUFFICIENT_SHMEM, "invalid gpu_method");
   if (opts_.gpu_method == 4)
     fail(FINUFFT_ERR_METHOD_NOTVALID, "block-gather (gpu_method=4) is not provided");
-  if (type_ == 2 && opts_.gpu_method == 3)
-    fail(FINUFFT_ERR_METHOD_NOTVALID, "output-driven (gpu_method=3) only spreads");
+  if (opts_.gpu_method == 3)
+    fail(FINUFFT_ERR_METHOD_NOTVALID, "output-driven spreading (gpu_method=3) is not provided");
 
   device_guard dg(opts_.gpu_device_id);
   upload_coefficients();
@@ -349,7 +355,14 @@ void Plan<T>::setup_bins() {
   // coordinate; the LDS methods bin by first support node.
   const int lds_limit = device_lds_limit(opts_.gpu_device_id);
   int *bs = geom_.binsize;
-  const int user[3] = {opts_.gpu_binsizex, opts_.gpu_binsizey, opts_.gpu_binsizez};
+  // The automatic method is chosen again for every grid (type 3 calls this
+  // at each setpts).
+  if (auto_method_) opts_.gpu_method = 2;
+  // User bin sizes are taken as given up to the grid size: a start-node bin
+  // wider than its grid would give a tile past the wrap range of the tile
+  // kernels (two conditional subtractions of nf).
+  int user[3];
+  for (int d = 0; d < 3; ++d) user[d] = std::min(user_bin_[d], nf_[d]);
   const bool user_bins = user[0] > 0 || user[1] > 0 || user[2] > 0;
   auto set_ref_bins = [&]() {
     const int b = dim_ == 1 ? 1024 : dim_ == 2 ? 40 : 8;
@@ -364,7 +377,8 @@ void Plan<T>::setup_bins() {
   // Column-window spreading: 3-D fp32 spreading plans with ns <= 8. User bin
   // sizes select it when x and y are 8 (the footprint of one wave).
   const bool col_ok = opts_.gpu_method != 1 && type_ != 2 && dim_ == 3 &&
-                      std::is_same<T, float>::value && ns_ <= COL_MAX_NS &&
+                      std::is_same<T, float>::value && ns_ <= COL_MAX_NS && nf_[0] >= 8 &&
+                      nf_[1] >= 8 &&
                       (!user_bins || (user[0] == 8 && user[1] == 8 && user[2] > 0));
   bool col_done = false;
   if (col_ok) {
@@ -397,7 +411,8 @@ void Plan<T>::setup_bins() {
     const size_t cell = spreads ? 2 * sizeof(double) : 2 * sizeof(T);
     bool ok = true;
     if (user_bins) {
-      for (int d = 0; d < 3; ++d) bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : 8) : 1;
+      for (int d = 0; d < 3; ++d)
+        bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : std::min(8, nf_[d])) : 1;
       ok = size_t(tile_cells(dim_, bs, ns_)) * cell <= size_t(lds_limit);
     } else {
       // about 1024 workgroups per launch (bins x vectors of a batch)
@@ -429,9 +444,6 @@ void Plan<T>::setup_bins() {
   for (int d = 0; d < 3; ++d)
     if (bs[d] < 1) fail(FINUFFT_ERR_BINSIZE_NOTVALID, "invalid bin size");
   if (opts_.gpu_maxsubprobsize < 1) opts_.gpu_maxsubprobsize = 1024;
-  opts_.gpu_binsizex = bs[0];
-  opts_.gpu_binsizey = bs[1];
-  opts_.gpu_binsizez = bs[2];
   geom_.dim = dim_;
   geom_.ns = ns_;
   geom_.fold64 = std::is_same<T, float>::value ? 1 : 0;
@@ -518,15 +530,24 @@ template <typename T>
 void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *s, const T *t,
                      const T *u) {
   device_guard dg(opts_.gpu_device_id);
+  ++setpts_gen_;
   if (M < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "negative number of points");
   if (M > 0 && (x == nullptr || (dim_ > 1 && y == nullptr) || (dim_ > 2 && z == nullptr)))
     fail(FINUFFT_ERR_INVALID_ARGUMENT, "null coordinate array");
   if (type_ != 3) {
     M_ = M;
+    if (M == 0) {
+      // No points: nothing is sorted and execute launches nothing but the
+      // clearing of a type-1 output.
+      for (int d = 0; d < 3; ++d) xyz_[d] = nullptr;
+      n_dense_sub_ = 0;
+      return;
+    }
     sort_points(x, y, z);
     return;
   }
   // ---- type 3 (setpts.cu:92-313)
+  t3_set_ = false;
   M_ = M;
   Nk_ = N;
   if (N < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: negative number of targets");
@@ -534,6 +555,13 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
   if (dim_ > 0 && s == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: s is null");
   if (dim_ > 1 && t == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: t is null");
   if (dim_ > 2 && u == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: u is null");
+  if (M == 0 || N == 0) {
+    // No sources (the output is zero) or no targets (there is no output):
+    // execute clears the output or returns, with no other launch.
+    inner_.reset();
+    t3_set_ = true;
+    return;
+  }
   const T *src[3] = {x, y, z};
   const T *trg[3] = {s, t, u};
   double h[3] = {1, 1, 1}, gam[3] = {1, 1, 1};
@@ -603,6 +631,7 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
   inner_.reset();
   inner_ = std::make_unique<Plan<T>>(2, dim_, nf_, iflag_, batchsize_, tol_, t2opts);
   inner_->setpts(N, stup_[0].get(), stup_[1].get(), stup_[2].get(), 0, nullptr, nullptr, nullptr);
+  t3_set_ = true;
 }
 
 // ------------------------------------------------------------ execute
@@ -647,6 +676,10 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
 
 template <typename T>
 void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
+  if (M_ == 0) {  // no points: the modes (or the spread grid) are zero
+    zero<T>(d_fk, nmodes_ * ntr, stream_);
+    return;
+  }
   const SpreadCtx<T> ctx = spread_ctx();
   const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
   for (int i = 0; i * batchsize_ < ntr; ++i) {
@@ -672,6 +705,7 @@ void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
 
 template <typename T>
 void Plan<T>::exec_type2(C *d_c, C *d_fk, int fft_sign, int ntr) {
+  if (M_ == 0) return;  // no points: no values to compute
   const SpreadCtx<T> ctx = spread_ctx();
   const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
   for (int i = 0; i * batchsize_ < ntr; ++i) {
@@ -696,6 +730,11 @@ void Plan<T>::exec_type2(C *d_c, C *d_fk, int fft_sign, int ntr) {
 
 template <typename T>
 void Plan<T>::exec_type3(C *d_c, C *d_fk) {
+  if (Nk_ == 0) return;  // no targets
+  if (M_ == 0) {         // no sources: the target values are zero
+    zero<T>(d_fk, int64_t(Nk_) * ntransf_, stream_);
+    return;
+  }
   const SpreadCtx<T> ctx = spread_ctx();
   for (int i = 0; i * batchsize_ < ntransf_; ++i) {
     const int blk = std::min(ntransf_ - i * batchsize_, batchsize_);
@@ -712,7 +751,7 @@ void Plan<T>::exec_type3(C *d_c, C *d_fk) {
 template <typename T>
 void Plan<T>::check_data(const C *c, const C *fk) const {
   // Null data arrays are refused where a kernel would read or write them.
-  if (type_ == 3 && !inner_)
+  if (type_ == 3 && !t3_set_)
     fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: setpts must come before execute");
   if (M_ > 0 && c == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "null c array");
   if ((type_ != 3 || Nk_ > 0) && fk == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "null fk array");
@@ -747,6 +786,7 @@ void Plan<T>::interp_grad(C *d_c, C *d_dc, C *d_fk) {
   if (spread_only_) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad needs the full transform");
   if (d_dc == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad: dc is null");
   if (d_fk == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "interp_grad: fk is null");
+  if (M_ == 0) return;  // no points: no values or derivatives
   const int fft_sign = type_ == 2 ? iflag_ : -iflag_;
   const SpreadCtx<T> ctx = spread_ctx();
   const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};

@@ -71,6 +71,12 @@ class Plan {
   int dense_subproblems() const { return n_dense_sub_; }
   hipStream_t stream() const { return stream_; }
   StageTimer &timer() { return timer_; }
+  // Number of setpts calls so far (a batch records it and refuses to run
+  // when the points have changed since), and the number of live batches
+  // holding the plan (destroy is refused while it is not zero).
+  uint64_t setpts_generation() const { return setpts_gen_; }
+  int batch_refs() const { return batch_refs_; }
+  void add_batch_ref(int d) { batch_refs_ += d; }
 
  private:
   void setup_kernel();
@@ -92,6 +98,7 @@ class Plan {
   hipStream_t stream_ = nullptr;
   bool spread_only_ = false;
   bool auto_method_ = false;
+  int user_bin_[3] = {0, 0, 0};  // the caller's gpu_binsize{x,y,z}
 
   // kernel
   int ns_ = 2;
@@ -134,8 +141,11 @@ class Plan {
   dbuf<C> prephase_, deconv_, cpre_;
   double t3C_[3] = {0, 0, 0}, t3D_[3] = {0, 0, 0};
   std::unique_ptr<Plan<T>> inner_;
+  bool t3_set_ = false;  // type 3: setpts has completed
 
   StageTimer timer_;
+  uint64_t setpts_gen_ = 0;
+  int batch_refs_ = 0;
 };
 
 }  // namespace fa

@@ -1,0 +1,148 @@
+// LDS and VALU throughput probes (measurement helpers for the bench).
+//
+// The spreading kernels keep their accumulators in LDS; which LDS operations
+// they use (float or double atomic adds, plain read-modify-write, uniform
+// broadcast reads) is decided by the throughputs measured here, with the
+// library's own compile flags. Each probe launches many workgroups of 256
+// lanes; every lane runs `reps` iterations of 8 operations of one kind at
+// addresses that are distinct within a wave (or uniform, for the broadcast
+// probes), so bank conflicts do not enter. The host side turns the elapsed
+// time into wave instructions per clock per CU.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace fa {
+namespace {
+
+constexpr int PROBE_T = 256;
+constexpr int PROBE_WORDS = 16384;  // 64 KiB of LDS
+
+enum ProbeMode {
+  P_ADD_F32 = 0,      // ds_add_f32, lane-distinct dwords
+  P_ADD_F64 = 1,      // ds_add_f64, lane-distinct qwords
+  P_ADD_U32 = 2,      // ds_add_u32, lane-distinct dwords
+  P_RMW_F32 = 3,      // ds_read_b32 + v_add + ds_write_b32 (wave-private cells)
+  P_READ_B32 = 4,     // ds_read_b32, lane-distinct
+  P_READ_B32_U = 5,   // ds_read_b32, one address per wave (broadcast)
+  P_READ_B128 = 6,    // ds_read_b128, lane-distinct
+  P_READ_B128_U = 7,  // ds_read_b128, one address per wave (broadcast)
+  P_FMA_F32 = 8,      // v_fma_f32 (VALU issue rate)
+  P_COUNT = 9
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(PROBE_T) lds_probe_kernel(float *out, int reps) {
+  __shared__ __align__(16) float t[PROBE_WORDS];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  for (int i = tid; i < PROBE_WORDS; i += PROBE_T) t[i] = float(i & 7);
+  __syncthreads();
+  float acc = 0.f;
+  if constexpr (MODE == P_ADD_F32) {
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) atomicAdd(&t[(tid + k * PROBE_T) & (PROBE_WORDS - 1)], 1.0f);
+  } else if constexpr (MODE == P_ADD_F64) {
+    double *d = reinterpret_cast<double *>(t);
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        atomicAdd(&d[(tid + k * PROBE_T) & (PROBE_WORDS / 2 - 1)], 1.0);
+  } else if constexpr (MODE == P_ADD_U32) {
+    unsigned *u = reinterpret_cast<unsigned *>(t);
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) atomicAdd(&u[(tid + k * PROBE_T) & (PROBE_WORDS - 1)], 1u);
+  } else if constexpr (MODE == P_RMW_F32) {
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float *p = &t[(tid + k * PROBE_T) & (PROBE_WORDS - 1)];
+        *p = *p + 1.0f;
+      }
+  } else if constexpr (MODE == P_READ_B32 || MODE == P_READ_B32_U) {
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int a = MODE == P_READ_B32 ? tid + k * PROBE_T : wave * 64 + k * 8 + (r & 7);
+        acc += t[(a + r) & (PROBE_WORDS - 1)];
+      }
+  } else if constexpr (MODE == P_READ_B128 || MODE == P_READ_B128_U) {
+    const float4 *v = reinterpret_cast<const float4 *>(t);
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int a = MODE == P_READ_B128 ? tid + k * PROBE_T : wave * 64 + k * 8 + (r & 7);
+        const float4 q = v[(a + r) & (PROBE_WORDS / 4 - 1)];
+        acc += (q.x + q.y) + (q.z + q.w);
+      }
+  } else {
+    float x = float(tid), y = 1.0001f;
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        x = std::fma(x, y, 0.5f);
+        y = std::fma(y, x, -0.25f);
+      }
+    acc = x + y;
+  }
+  __syncthreads();
+  acc += t[tid];
+  out[blockIdx.x * PROBE_T + tid] = acc;
+}
+
+template <int MODE>
+void launch_probe(float *out, int blocks, int reps, hipStream_t s) {
+  lds_probe_kernel<MODE><<<blocks, PROBE_T, 0, s>>>(out, reps);
+  launch_check("lds_probe_kernel");
+}
+
+}  // namespace
+
+// Runs probe `mode` over `blocks` workgroups of 256 lanes with `reps`
+// iterations of 8 operations each; returns the elapsed milliseconds and the
+// number of wave-level operations issued (the FMA probe counts two per
+// iteration step).
+void lds_probe(int mode, int blocks, int reps, double *ms, double *wave_ops) {
+  if (mode < 0 || mode >= P_COUNT) fail(FINUFFT_ERR_INVALID_ARGUMENT, "lds_probe: bad mode");
+  if (blocks < 1 || reps < 1) fail(FINUFFT_ERR_INVALID_ARGUMENT, "lds_probe: bad size");
+  dbuf<float> out(size_t(blocks) * PROBE_T);
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hip_check(hipEventCreate(&b), "hipEventCreate");
+  auto run = [&](int r) {
+    switch (mode) {
+      case P_ADD_F32: return launch_probe<P_ADD_F32>(out.get(), blocks, r, nullptr);
+      case P_ADD_F64: return launch_probe<P_ADD_F64>(out.get(), blocks, r, nullptr);
+      case P_ADD_U32: return launch_probe<P_ADD_U32>(out.get(), blocks, r, nullptr);
+      case P_RMW_F32: return launch_probe<P_RMW_F32>(out.get(), blocks, r, nullptr);
+      case P_READ_B32: return launch_probe<P_READ_B32>(out.get(), blocks, r, nullptr);
+      case P_READ_B32_U: return launch_probe<P_READ_B32_U>(out.get(), blocks, r, nullptr);
+      case P_READ_B128: return launch_probe<P_READ_B128>(out.get(), blocks, r, nullptr);
+      case P_READ_B128_U: return launch_probe<P_READ_B128_U>(out.get(), blocks, r, nullptr);
+      default: return launch_probe<P_FMA_F32>(out.get(), blocks, r, nullptr);
+    }
+  };
+  run(1);  // warm-up
+  // the loop-free part (LDS fill, final store) is timed with reps = 1 and
+  // subtracted
+  float t1 = 0, tr = 0;
+  hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
+  run(1);
+  hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
+  hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+  hip_check(hipEventElapsedTime(&t1, a, b), "hipEventElapsedTime");
+  hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
+  run(reps);
+  hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
+  hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+  hip_check(hipEventElapsedTime(&tr, a, b), "hipEventElapsedTime");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = double(tr) - double(t1);
+  const double per_it = mode == P_FMA_F32 ? 16.0 : 8.0;
+  *wave_ops = double(blocks) * (PROBE_T / 64) * double(reps - 1) * per_it;
+}
+
+}  // namespace fa

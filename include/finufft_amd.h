@@ -114,6 +114,18 @@ FINUFFT_EXPORT int finufft_amd_copy_sort(const void *plan, int is_double, int *p
 FINUFFT_EXPORT int finufft_amd_set_stage_timing(void *plan, int is_double, int on);
 FINUFFT_EXPORT int finufft_amd_get_stage_times(void *plan, int is_double, double *ms);
 
+/* LDS / VALU throughput probe on the current device; no reference
+ * counterpart (a measurement helper: the spreading kernels' choice of LDS
+ * operations rests on these rates). mode: 0 ds_add_f32, 1 ds_add_f64,
+ * 2 ds_add_u32, 3 float read-modify-write (ds_read + v_add + ds_write),
+ * 4 ds_read_b32, 5 ds_read_b32 at one address per wave, 6 ds_read_b128,
+ * 7 ds_read_b128 at one address per wave, 8 v_fma_f32. Runs blocks
+ * workgroups of 256 lanes, reps iterations each; writes the elapsed
+ * milliseconds (the single-iteration launch subtracted) and the number of
+ * wave-level operations timed. */
+FINUFFT_EXPORT int finufft_amd_lds_probe(int mode, int blocks, int reps, double *ms,
+                                         double *wave_ops);
+
 /* ---- plan-time math, exposed for tests and tools (host only, no GPU) ---- */
 
 /* Smallest 2,3,5-smooth multiple of f that is >= n; reference:
