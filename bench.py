@@ -246,7 +246,8 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
 
 
 PROBE_MODES = ("ds_add_f32", "ds_add_f64", "ds_add_u32", "rmw_f32", "ds_read_b32",
-               "ds_read_b32_uniform", "ds_read_b128", "ds_read_b128_uniform", "v_fma_f32")
+               "ds_read_b32_uniform", "ds_read_b128", "ds_read_b128_uniform", "v_fma_f32",
+               "ds_add_f32_rotating", "ds_add_f64_rotating")
 
 
 def lds_probe():

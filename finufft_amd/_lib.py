@@ -107,10 +107,15 @@ def _declare(lib):
     lib.finufft_amd_gaussquad.argtypes = [c_int, POINTER(c_double), POINTER(c_double)]
     lib.finufft_amd_gaussquad.restype = None
     lib.finufft_amd_kernel_shape.argtypes = [
-        c_double, c_int, c_double, c_int, POINTER(c_int), POINTER(c_double)]
+        c_double, c_int, c_double, c_int, c_int, c_int, POINTER(c_int), POINTER(c_double),
+        POINTER(c_int)]
     lib.finufft_amd_kernel_shape.restype = c_int
     lib.finufft_amd_horner_coeffs.argtypes = [c_int, c_double, POINTER(c_double), c_int]
     lib.finufft_amd_horner_coeffs.restype = c_int
+    lib.finufft_amd_horner_coeffs_pswf.argtypes = [c_int, c_double, POINTER(c_double), c_int]
+    lib.finufft_amd_horner_coeffs_pswf.restype = c_int
+    lib.finufft_amd_pswf.argtypes = [c_double, vp, c_int64, vp]
+    lib.finufft_amd_pswf.restype = None
     lib.finufft_amd_fold_rescale_f.argtypes = [vp, c_int64, c_int, vp]
     lib.finufft_amd_fold_rescale_f.restype = None
     lib.finufft_amd_fold_rescale_d.argtypes = [vp, c_int64, c_int, vp]

@@ -425,22 +425,28 @@ void finufft_amd_gaussquad(int n, double *x, double *w) {
 }
 
 int finufft_amd_kernel_shape(double tol, int is_double, double upsampfac, int kerevalmeth,
-                             int *ns, double *beta) {
+                             int dim, int type, int *ns, double *beta, int *formula) {
   return safe_call([&] {
+    if (kerevalmeth < 0 || kerevalmeth > 2)
+      fa::fail(FINUFFT_ERR_KERFORMULA_NOTVALID, "gpu_kerevalmeth must be 0, 1 or 2");
     if (upsampfac != 2.0 && upsampfac != 1.25) {
-      if (kerevalmeth == 1) fa::fail(FINUFFT_ERR_HORNER_WRONG_BETA, "Horner needs sigma 2 or 1.25");
+      if (kerevalmeth != 0) fa::fail(FINUFFT_ERR_HORNER_WRONG_BETA, "Horner needs sigma 2 or 1.25");
       if (upsampfac <= 1.0) fa::fail(FINUFFT_ERR_UPSAMPFAC_TOO_SMALL, "upsampfac must exceed 1");
     }
     fa::KernelShape ks;
     if (is_double) {
       const double e = std::max(tol, std::numeric_limits<double>::epsilon());
-      ks = fa::kernel_shape(e, e / 10.0, upsampfac, fa::MAX_NS);
+      ks = kerevalmeth == 1 ? fa::kernel_shape_pswf(e, upsampfac, dim, type, fa::MAX_NS, false)
+                            : fa::kernel_shape(e, e / 10.0, upsampfac, fa::MAX_NS);
     } else {
       const float e = std::max(float(tol), std::numeric_limits<float>::epsilon());
-      ks = fa::kernel_shape(double(e), double(e / 10.0f), upsampfac, fa::MAX_NS_F32);
+      ks = kerevalmeth == 1
+               ? fa::kernel_shape_pswf(double(e), upsampfac, dim, type, fa::MAX_NS_F32, true)
+               : fa::kernel_shape(double(e), double(e / 10.0f), upsampfac, fa::MAX_NS_F32);
     }
     if (ns) *ns = ks.ns;
     if (beta) *beta = ks.beta;
+    if (formula) *formula = ks.formula;
   });
 }
 
@@ -452,6 +458,20 @@ int finufft_amd_horner_coeffs(int ns, double upsampfac, double *out, int nc_max)
   const std::vector<double> c = fa::horner_coeffs(ns, upsampfac, bo * ns, nc);
   std::copy(c.begin(), c.end(), out);
   return nc;
+}
+
+int finufft_amd_horner_coeffs_pswf(int ns, double beta, double *out, int nc_max) {
+  if (ns < 2 || ns > fa::MAX_NS || !out || beta <= 0) return 0;
+  const int nc = std::min(nc_max, fa::horner_nc_sigma2(ns));
+  const std::vector<double> c = fa::horner_coeffs_pswf(ns, beta, nc);
+  std::copy(c.begin(), c.end(), out);
+  return nc;
+}
+
+void finufft_amd_pswf(double c, const double *z, int64_t n, double *out) {
+  if (!z || !out || n <= 0 || !(c > 0)) return;
+  const fa::Prolate0 psi(c);
+  for (int64_t i = 0; i < n; ++i) out[i] = psi(z[i]);
 }
 
 void finufft_amd_fold_rescale_f(const float *x, int64_t m, int n, float *out) {

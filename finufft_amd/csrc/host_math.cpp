@@ -29,7 +29,99 @@ KernelShape kernel_shape(double eps, double eps_div10, double upsampfac, int max
   if (ns == 4) betaoverns = 2.38;
   if (upsampfac != 2.0) betaoverns = 0.97 * PI * (1.0 - 1.0 / (2.0 * upsampfac));
   ks.beta = betaoverns * ns;
+  ks.formula = KER_ES;
   return ks;
+}
+
+KernelShape kernel_shape_pswf(double eps, double upsampfac, int dim, int type, int max_ns,
+                              bool is_float) {
+  KernelShape ks{};
+  // Aliasing law tol = tolfac exp(-(ns - 1) pi sqrt(1 - 1/sigma)) with the
+  // reference's empirical per-dimension and type-3 factors.
+  double tolfac = 0.18;
+  for (int d = 1; d < dim; ++d) tolfac *= 1.4;
+  if (type == 3) tolfac *= 1.4;
+  int ns = int(std::ceil(std::log(tolfac / eps) / (PI * std::sqrt(1.0 - 1.0 / upsampfac)) + 1.0));
+  ns = std::max(2, ns);
+  ks.clipped = ns > max_ns;
+  ns = std::min(ns, max_ns);
+  // fp32 at low upsampling loses digits to cancellation beyond width 8
+  if (is_float && upsampfac < 1.4) ns = std::min(ns, 8);
+  ks.ns = ns;
+  ks.beta = PI * ns * (1.0 - 1.0 / (2.0 * upsampfac)) - 0.05;
+  ks.formula = KER_PSWF;
+  return ks;
+}
+
+Prolate0::Prolate0(double c) {
+  // Even Legendre degrees k = 2m, m < n; n covers the eigenvector's decay
+  // (its coefficients fall off like (c / 2k)^k) with a wide margin.
+  const int n = int(c) + 40;
+  std::vector<long double> d(n), e(n, 0.0L);
+  const long double cc = (long double)c * c;
+  for (int m = 0; m < n; ++m) {
+    const long double k = 2.0L * m;
+    // x^2 P_k = A P_{k+2} + B P_k + C P_{k-2}; in the orthonormal basis the
+    // operator is k(k+1) + c^2 B on the diagonal and c^2 A sqrt((2k+1)/(2k+5))
+    // between k and k+2 (symmetric).
+    long double B = (k + 1) * (k + 1) / ((2 * k + 1) * (2 * k + 3));
+    if (m > 0) B += k * k / ((2 * k + 1) * (2 * k - 1));
+    d[m] = k * (k + 1) + cc * B;
+    e[m] = cc * (k + 1) * (k + 2) / ((2 * k + 3) * std::sqrt((2 * k + 1) * (2 * k + 5)));
+  }
+  // Inverse iteration with zero shift: the matrix is positive definite and
+  // psi_0 is its lowest eigenvector; each step solves the tridiagonal system
+  // by elimination without pivoting (stable for SPD matrices).
+  std::vector<long double> v(n, 0.0L), w(n), cp(n), dp(n);
+  v[0] = 1.0L;
+  for (int it = 0; it < 300; ++it) {
+    cp[0] = e[0] / d[0];
+    dp[0] = v[0] / d[0];
+    for (int m = 1; m < n; ++m) {
+      const long double den = d[m] - e[m - 1] * cp[m - 1];
+      cp[m] = e[m] / den;
+      dp[m] = (v[m] - e[m - 1] * dp[m - 1]) / den;
+    }
+    w[n - 1] = dp[n - 1];
+    for (int m = n - 2; m >= 0; --m) w[m] = dp[m] - cp[m] * w[m + 1];
+    long double nrm = 0;
+    for (int m = 0; m < n; ++m) nrm += w[m] * w[m];
+    nrm = std::sqrt(nrm);
+    for (int m = 0; m < n; ++m) v[m] = w[m] / nrm;
+  }
+  a_.resize(n);
+  for (int m = 0; m < n; ++m) a_[m] = v[m] * std::sqrt(2.0L * m + 0.5L);
+  scale_ = 1.0L / raw(0.0L);
+}
+
+long double Prolate0::raw(long double x) const {
+  // sum_m a_m P_2m(x) with the three-term recurrence of P_k
+  long double p0 = 1, p1 = x, s = a_[0];
+  const int kmax = 2 * int(a_.size()) - 2;
+  for (int k = 1; k < kmax; ++k) {
+    const long double p2 = ((2 * k + 1) * x * p1 - k * p0) / (k + 1);
+    p0 = p1;
+    p1 = p2;
+    if ((k + 1) % 2 == 0) s += a_[(k + 1) / 2] * p1;
+  }
+  return s;
+}
+
+double Prolate0::operator()(double x) const {
+  if (std::abs(x) > 1.0) return 0.0;
+  return double(raw(x) * scale_);
+}
+
+KernelFn::KernelFn(int formula, int ns, double beta) : formula_(formula), ns_(ns), beta_(beta) {
+  if (formula_ == KER_PSWF) pswf_.emplace_back(beta);
+}
+
+double KernelFn::operator()(double x) const {
+  if (formula_ == KER_PSWF) {
+    const double z = 2.0 * x / ns_;
+    return std::abs(z) >= 1.0 ? 0.0 : pswf_[0](z);
+  }
+  return es_kernel(x, ns_, beta_);
 }
 
 double es_kernel(double x, int ns, double beta) {
@@ -135,6 +227,32 @@ std::vector<double> horner_coeffs(int ns, double upsampfac, double beta, int nc_
   return out;
 }
 
+std::vector<double> horner_coeffs_pswf(int ns, double beta, int nc_out) {
+  const Prolate0 psi(beta);
+  const int npts = 4 * nc_out, cols = nc_out;
+  std::vector<long double> A(size_t(npts) * cols), B(size_t(npts) * ns);
+  for (int p = 0; p < npts; ++p) {
+    // Chebyshev points of the first kind on [-1, 1]
+    const long double t = std::cos(PI * (2 * p + 1) / (2.0 * npts));
+    long double pw = 1;
+    for (int k = 0; k < cols; ++k) {
+      A[size_t(p) * cols + k] = pw;
+      pw *= t;
+    }
+    for (int i = 0; i < ns; ++i) {
+      // interval i of the support [-1, 1], its local variable t
+      const double z = (-1.0 + (2.0 * i + 1.0) / ns) + double(t) / ns;
+      B[size_t(p) * ns + i] = psi(z);
+    }
+  }
+  std::vector<long double> X;
+  lstsq_householder(npts, cols, A, ns, B, X);
+  std::vector<double> out(size_t(nc_out) * ns, 0.0);
+  for (int k = 0; k < nc_out; ++k)
+    for (int i = 0; i < ns; ++i) out[size_t(k) * ns + i] = double(X[size_t(k) * ns + i]);
+  return out;
+}
+
 int64_t next235(int64_t n, int64_t required_factor) {
   n = std::max<int64_t>(n, 1);
   required_factor = std::max<int64_t>(required_factor, 1);
@@ -199,14 +317,14 @@ int64_t fine_grid_len(int64_t ms, double upsampfac, int ns) {
   return nf;
 }
 
-void fourier_nodes(int ns, double beta, bool type3, int *q, double *z, double *f) {
+void fourier_nodes(const KernelFn &ker, int ns, bool type3, int *q, double *z, double *f) {
   const double J2 = ns / 2.0;
   const int qq = type3 ? int(2 + 2.0 * J2) : int(2 + 3.0 * J2);
   std::vector<double> xg(2 * qq), wg(2 * qq);
   gaussquad(2 * qq, xg.data(), wg.data());
   for (int n = 0; n < qq; ++n) {
     z[n] = J2 * xg[n];  // nodes in (-ns/2, 0); the kernel is even
-    f[n] = J2 * wg[n] * es_kernel(z[n], ns, beta);
+    f[n] = J2 * wg[n] * ker(z[n]);
   }
   *q = qq;
 }

@@ -161,22 +161,23 @@ bool choose_bins(int dim, const int *nf, int ns, size_t cell_bytes, size_t budge
   return found;
 }
 
-// Bin shape of the column-window spreader (3-D, fp32, ns <= 8): 8 x 8 first
-// nodes in (x, y) (a multiple of every sub-column side) and the tallest z
-// extent whose tile plus the staging area fits, in order of preference:
+// Bin shape of the column spreader (3-D, fp32, ns <= 8): 8 x 8 first nodes in
+// (x, y) (a multiple of every sub-column side) and the tallest z extent up to
+// COL_ZC (the register depth of a column) whose tile plus the staging area
+// fits, in order of preference:
 // colorable within `budget` (two workgroups per CU), colorable within the
 // device limit, any shape within `budget`, any within the limit. Returns
 // false when nothing fits.
 bool choose_col_bins(const int *nf, int ns, size_t budget, size_t limit, int *out,
                      bool *colored) {
-  static const int zc[] = {32, 24, 16, 12, 8, 6, 4, 2, 1};
+  static const int zc[] = {16, 12, 8, 6, 4, 2, 1};
   for (int pass = 0; pass < 4; ++pass) {
     const bool want_col = pass < 2;
     const int64_t cap = int64_t(pass % 2 == 0 ? budget : limit);
     for (int bz : zc) {
-      if (bz > nf[2]) continue;  // a bin never exceeds its grid
+      if (bz > nf[2] || bz > COL_ZC) continue;  // a bin never exceeds its grid
       const int b[3] = {8, 8, bz};
-      if (tile_cells(3, b, ns) * 16 + int64_t(col_stage_bytes()) > cap) continue;
+      if (int64_t(col_lds_bytes(tile_cells(3, b, ns))) > cap) continue;
       const bool colr = colorable(3, nf, b, ns);
       if (want_col && !colr) continue;
       out[0] = 8;
@@ -308,12 +309,19 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
 
 template <typename T>
 void Plan<T>::setup_kernel() {
-  // setup_spreadinterp (makeplan.cu:43-119)
+  // setup_spreadinterp (makeplan.cu:43-119). gpu_kerevalmeth selects the
+  // kernel: 1 (the default) Horner polynomials of the PSWF kernel with the
+  // reference CPU library's width law (its v2.5 kernel, which the GPU
+  // library's setup_spreadinterp lists as its own to-do, makeplan.cu:52);
+  // 0 the ES kernel evaluated directly and 2 the ES kernel's Horner
+  // polynomials, both with the GPU library's width rule.
   const double sigma = opts_.upsampfac;
+  const int meth = opts_.gpu_kerevalmeth;
+  if (meth < 0 || meth > 2) fail(FINUFFT_ERR_KERFORMULA_NOTVALID, "gpu_kerevalmeth must be 0, 1 or 2");
   if (sigma != 2.0 && sigma != 1.25) {
-    if (opts_.gpu_kerevalmeth == 1)
+    if (meth != 0)
       fail(FINUFFT_ERR_HORNER_WRONG_BETA,
-           "nonstandard upsampfac cannot use the Horner kernel (kerevalmeth=1)");
+           "nonstandard upsampfac cannot use the Horner kernel (kerevalmeth=1 or 2)");
     if (sigma <= 1.0) fail(FINUFFT_ERR_UPSAMPFAC_TOO_SMALL, "upsampfac must exceed 1");
     if (!opts_.gpu_spreadinterponly && sigma > 4.0 && opts_.debug)
       std::fprintf(stderr, "[finufft_amd] warning: upsampfac=%.3g is too large to be beneficial\n",
@@ -327,21 +335,27 @@ void Plan<T>::setup_kernel() {
                    double(eps), double(eps_mach));
     eps = eps_mach;
   }
-  const KernelShape ks = kernel_shape(double(eps), double(eps / T(10)), sigma, max_ns_of<T>());
+  const KernelShape ks =
+      meth == 1 ? kernel_shape_pswf(double(eps), sigma, dim_, type_, max_ns_of<T>(),
+                                    std::is_same<T, float>::value)
+                : kernel_shape(double(eps), double(eps / T(10)), sigma, max_ns_of<T>());
   if (ks.clipped && opts_.debug)
     std::fprintf(stderr, "[finufft_amd] warning: kernel width clipped to %d\n", ks.ns);
   ns_ = ks.ns;
   beta_ = ks.beta;
+  formula_ = ks.formula;
   if (opts_.debug)
-    std::printf("[finufft_amd] kerevalmeth=%d tol=%.3g sigma=%.3g: ns=%d beta=%.4g\n",
-                opts_.gpu_kerevalmeth, double(eps), sigma, ns_, beta_);
+    std::printf("[finufft_amd] kerevalmeth=%d tol=%.3g sigma=%.3g: %s ns=%d beta=%.4g\n", meth,
+                double(eps), sigma, formula_ == KER_PSWF ? "PSWF" : "ES", ns_, beta_);
 }
 
 template <typename T>
 void Plan<T>::upload_coefficients() {
-  if (opts_.gpu_kerevalmeth != 1) return;
+  if (opts_.gpu_kerevalmeth == 0) return;
   const int nc = horner_nc_sigma2(ns_);
-  const std::vector<double> c64 = horner_coeffs(ns_, opts_.upsampfac, beta_, nc);
+  const std::vector<double> c64 = formula_ == KER_PSWF
+                                      ? horner_coeffs_pswf(ns_, beta_, nc)
+                                      : horner_coeffs(ns_, opts_.upsampfac, beta_, nc);
   std::vector<T> c(c64.begin(), c64.end());
   coef_.resize(c.size());
   hip_check(hipMemcpy(coef_.get(), c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice),
@@ -374,12 +388,14 @@ void Plan<T>::setup_bins() {
   };
   geom_.colS = 0;
   geom_.bank = 0;
-  // Column-window spreading: 3-D fp32 spreading plans with ns <= 8. User bin
-  // sizes select it when x and y are 8 (the footprint of one wave).
+  // Column spreading: 3-D fp32 spreading plans with ns <= 8. User bin sizes
+  // select it when x and y are 8 (the footprint of one wave) and z is at most
+  // COL_ZC.
   const bool col_ok = opts_.gpu_method != 1 && type_ != 2 && dim_ == 3 &&
                       std::is_same<T, float>::value && ns_ <= COL_MAX_NS && nf_[0] >= 8 &&
                       nf_[1] >= 8 &&
-                      (!user_bins || (user[0] == 8 && user[1] == 8 && user[2] > 0));
+                      (!user_bins ||
+                       (user[0] == 8 && user[1] == 8 && user[2] > 0 && user[2] <= COL_ZC));
   bool col_done = false;
   if (col_ok) {
     const size_t budget = std::min<size_t>(size_t(lds_limit), 80 * 1024);
@@ -388,7 +404,7 @@ void Plan<T>::setup_bins() {
       bs[0] = bs[1] = 8;
       bs[2] = user[2];
       colr = colorable(3, nf_, bs, ns_);
-      col_done = tile_cells(3, bs, ns_) * 16 + int64_t(col_stage_bytes()) <= lds_limit;
+      col_done = int64_t(col_lds_bytes(tile_cells(3, bs, ns_))) <= lds_limit;
     } else {
       col_done = choose_col_bins(nf_, ns_, budget, size_t(lds_limit), bs, &colr);
     }
@@ -396,7 +412,7 @@ void Plan<T>::setup_bins() {
       geom_.start_bins = 1;
       geom_.colS = fa::col_side(ns_);
       colored_ = colr;
-      lds_spread_ = size_t(tile_cells(3, bs, ns_)) * 16 + col_stage_bytes();
+      lds_spread_ = col_lds_bytes(tile_cells(3, bs, ns_));
       lds_bytes_ = size_t(tile_cells(3, bs, ns_)) * 2 * sizeof(T);
     }
   }
@@ -463,8 +479,9 @@ void Plan<T>::prepare_grid() {
     fft_.setup(dim_, nf_, batchsize_, std::is_same<T, double>::value);
     int q = 0;
     std::vector<double> z(3 * MAX_NQUAD, 0.0), f(3 * MAX_NQUAD, 0.0);
+    const KernelFn ker(formula_, ns_, beta_);
     for (int d = 0; d < dim_; ++d)
-      fourier_nodes(ns_, beta_, false, &q, z.data() + d * MAX_NQUAD, f.data() + d * MAX_NQUAD);
+      fourier_nodes(ker, ns_, false, &q, z.data() + d * MAX_NQUAD, f.data() + d * MAX_NQUAD);
     dbuf<double> dz(z.size()), df(f.size());
     hip_check(hipMemcpy(dz.get(), z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice),
               "hipMemcpy(nodes)");
@@ -603,7 +620,7 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
   {
     int q = 0;
     std::vector<double> zq(MAX_NQUAD), fq(MAX_NQUAD);
-    fourier_nodes(ns_, beta_, true, &q, zq.data(), fq.data());
+    fourier_nodes(KernelFn(formula_, ns_, beta_), ns_, true, &q, zq.data(), fq.data());
     dbuf<double> dz(MAX_NQUAD), df(MAX_NQUAD);
     hip_check(hipMemcpy(dz.get(), zq.data(), MAX_NQUAD * sizeof(double), hipMemcpyHostToDevice),
               "hipMemcpy(t3 nodes)");
@@ -653,7 +670,7 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   }
   c.kp.beta = T(beta_);
   c.kp.es_c = T(4.0 / (double(ns_) * ns_));
-  c.kp.horner = opts_.gpu_kerevalmeth == 1 ? 1 : 0;
+  c.kp.horner = opts_.gpu_kerevalmeth != 0 ? 1 : 0;
   c.kp.fold64 = std::is_same<T, float>::value ? 1 : 0;
   c.kp.coef = coef_.get();
   c.idx = idx_.get();

@@ -57,6 +57,7 @@ class Plan {
   int dim() const { return dim_; }
   int ns() const { return ns_; }
   double beta() const { return beta_; }
+  int kernel_formula() const { return formula_; }
   double upsampfac() const { return opts_.upsampfac; }
   int method() const { return opts_.gpu_method; }
   int nf(int d) const { return nf_[d]; }
@@ -103,6 +104,7 @@ class Plan {
   // kernel
   int ns_ = 2;
   double beta_ = 0;
+  int formula_ = 0;  // KernelFormula of the spreading kernel
   dbuf<T> coef_;
 
   // sizes

@@ -6,7 +6,9 @@
 // library's own compile flags. Each probe launches many workgroups of 256
 // lanes; every lane runs `reps` iterations of 8 operations of one kind at
 // addresses that are distinct within a wave (or uniform, for the broadcast
-// probes), so bank conflicts do not enter. The host side turns the elapsed
+// probes), so bank conflicts do not enter. The plain atomic probes revisit
+// each lane's 8 addresses every iteration; the rotating ones spread a lane's
+// updates over 64 (fp32) or 32 (fp64) addresses. The host side turns the elapsed
 // time into wave instructions per clock per CU.
 #include <hip/hip_runtime.h>
 
@@ -28,7 +30,9 @@ enum ProbeMode {
   P_READ_B128 = 6,    // ds_read_b128, lane-distinct
   P_READ_B128_U = 7,  // ds_read_b128, one address per wave (broadcast)
   P_FMA_F32 = 8,      // v_fma_f32 (VALU issue rate)
-  P_COUNT = 9
+  P_ADD_F32_ROT = 9,  // ds_add_f32, each lane cycling through 64 addresses
+  P_ADD_F64_ROT = 10, // ds_add_f64, each lane cycling through 32 addresses
+  P_COUNT = 11
 };
 
 template <int MODE>
@@ -43,6 +47,17 @@ __global__ void __launch_bounds__(PROBE_T) lds_probe_kernel(float *out, int reps
     for (int r = 0; r < reps; ++r)
 #pragma unroll
       for (int k = 0; k < 8; ++k) atomicAdd(&t[(tid + k * PROBE_T) & (PROBE_WORDS - 1)], 1.0f);
+  } else if constexpr (MODE == P_ADD_F32_ROT) {
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        atomicAdd(&t[(tid + k * PROBE_T + (r & 7) * 8 * PROBE_T) & (PROBE_WORDS - 1)], 1.0f);
+  } else if constexpr (MODE == P_ADD_F64_ROT) {
+    double *d = reinterpret_cast<double *>(t);
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        atomicAdd(&d[(tid + k * PROBE_T + (r & 3) * 8 * PROBE_T) & (PROBE_WORDS / 2 - 1)], 1.0);
   } else if constexpr (MODE == P_ADD_F64) {
     double *d = reinterpret_cast<double *>(t);
     for (int r = 0; r < reps; ++r)
@@ -121,6 +136,8 @@ void lds_probe(int mode, int blocks, int reps, double *ms, double *wave_ops) {
       case P_READ_B32_U: return launch_probe<P_READ_B32_U>(out.get(), blocks, r, nullptr);
       case P_READ_B128: return launch_probe<P_READ_B128>(out.get(), blocks, r, nullptr);
       case P_READ_B128_U: return launch_probe<P_READ_B128_U>(out.get(), blocks, r, nullptr);
+      case P_ADD_F32_ROT: return launch_probe<P_ADD_F32_ROT>(out.get(), blocks, r, nullptr);
+      case P_ADD_F64_ROT: return launch_probe<P_ADD_F64_ROT>(out.get(), blocks, r, nullptr);
       default: return launch_probe<P_FMA_F32>(out.get(), blocks, r, nullptr);
     }
   };

@@ -262,6 +262,114 @@ struct StartTile {
   }
 };
 
+// Walks a tile in steps of NT cells: each thread's first cell and the step in
+// tile coordinates are found once per unit, so that the walk needs no
+// division per cell.
+template <int DIM, int NT>
+struct TileWalk {
+  int x0, y0, z0, sx, sy, sz;
+  __device__ void init(const int *P) {
+    int r = threadIdx.x;
+    x0 = r % P[0];
+    r /= P[0];
+    y0 = DIM > 1 ? r % P[1] : 0;
+    z0 = DIM > 1 ? r / P[1] : 0;
+    r = NT;
+    sx = r % P[0];
+    r /= P[0];
+    sy = DIM > 1 ? r % P[1] : 0;
+    sz = DIM > 1 ? r / P[1] : 0;
+  }
+  // Grid index of tile cell (x, y, z) of a tile at off (< 2^31: the plan
+  // caps a vector's grid there), then the next cell of the walk.
+  template <typename T>
+  __device__ __forceinline__ int index_and_step(const SpreadCtx<T> &ctx, const int *P,
+                                                const int *off, int &x, int &y, int &z) const {
+    int gi = wrap_grid(x + off[0], ctx.nf[0]);
+    if (DIM > 1) {
+      int row = wrap_grid(y + off[1], ctx.nf[1]);
+      if (DIM > 2) row += wrap_grid(z + off[2], ctx.nf[2]) * ctx.nf[1];
+      gi += row * ctx.nf[0];
+    }
+    x += sx;
+    if (x >= P[0]) {
+      x -= P[0];
+      ++y;
+    }
+    if (DIM > 1) {
+      y += sy;
+      if (y >= P[1]) {
+        y -= P[1];
+        ++z;
+      }
+      z += sz;
+    }
+    return gi;
+  }
+};
+
+// Cells per thread whose global accesses a tile transfer keeps in flight:
+// the loads of a chunk are all issued before any of them is used (a loop of
+// single read-modify-writes would wait a full memory latency per cell,
+// since a load may not pass an earlier store to the same array).
+constexpr int TILE_CHUNK = 16;
+
+// Adds an fp64 LDS tile (re, im planes) to grid g. plain: read-modify-write
+// (the unit owns these cells for the launch: colored bins); otherwise
+// hardware atomics (zero cells skipped).
+template <int DIM, int NT, bool PLAIN, typename T>
+__device__ __forceinline__ void tile_to_grid(const SpreadCtx<T> &ctx, const int *P,
+                                             const int *off, int ntile, const double *re,
+                                             const double *im, cplx_t<T> *__restrict__ g) {
+  TileWalk<DIM, NT> w;
+  w.init(P);
+  int x = w.x0, y = w.y0, z = w.z0;
+  for (int base = threadIdx.x; base < ntile; base += TILE_CHUNK * NT) {
+    int gi[TILE_CHUNK];
+    cplx_t<T> v[TILE_CHUNK];
+#pragma unroll
+    for (int u = 0; u < TILE_CHUNK; ++u) {
+      gi[u] = w.index_and_step(ctx, P, off, x, y, z);
+      if (PLAIN && base + u * NT < ntile) v[u] = g[gi[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < TILE_CHUNK; ++u) {
+      const int n = base + u * NT;
+      if (n < ntile) {
+        const T vr = T(re[n]), vi = T(im[n]);
+        if (PLAIN) {
+          v[u].x += vr;
+          v[u].y += vi;
+          g[gi[u]] = v[u];
+        } else if (vr != T(0) || vi != T(0)) {
+          hbm_add<T>(g + gi[u], vr, vi);
+        }
+      }
+    }
+  }
+}
+
+// Stages the tile of grid g into LDS (interleaved complex).
+template <int DIM, int NT, typename T>
+__device__ __forceinline__ void grid_to_tile(const SpreadCtx<T> &ctx, const int *P,
+                                             const int *off, int ntile,
+                                             const cplx_t<T> *__restrict__ g, cplx_t<T> *tile) {
+  TileWalk<DIM, NT> w;
+  w.init(P);
+  int x = w.x0, y = w.y0, z = w.z0;
+  for (int base = threadIdx.x; base < ntile; base += TILE_CHUNK * NT) {
+    cplx_t<T> v[TILE_CHUNK];
+#pragma unroll
+    for (int u = 0; u < TILE_CHUNK; ++u) {
+      const int gi = w.index_and_step(ctx, P, off, x, y, z);
+      if (base + u * NT < ntile) v[u] = g[gi];
+    }
+#pragma unroll
+    for (int u = 0; u < TILE_CHUNK; ++u)
+      if (base + u * NT < ntile) tile[base + u * NT] = v[u];
+  }
+}
+
 // Spreads the unit's points into the LDS tile held as two fp64 planes.
 // Accumulating in fp64 is what makes LDS spreading fast on this GPU: its fp64
 // LDS atomic add runs about 20x faster than the fp32 one, and the sums are
@@ -354,14 +462,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
   __syncthreads();
   spread_into_tile<T, DIM, NS>(ctx, st, c + int64_t(t) * ctx.M, re, im);
   __syncthreads();
-  cplx_t<T> *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
-    const int64_t gi = st.grid_index(ctx, n);
-    cplx_t<T> v = g[gi];
-    v.x += T(re[n]);
-    v.y += T(im[n]);
-    g[gi] = v;
-  }
+  tile_to_grid<DIM, BIN_THREADS, true>(ctx, st.P, st.off, st.ntile, re, im,
+                                       fw + int64_t(t) * nftot);
 }
 
 // Subproblem spreading with an atomic flush (any geometry, any density).
@@ -382,253 +484,242 @@ __global__ void __launch_bounds__(BIN_THREADS)
   __syncthreads();
   spread_into_tile<T, DIM, NS>(ctx, st, c + int64_t(t) * ctx.M, re, im);
   __syncthreads();
-  cplx_t<T> *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
-    const T vr = T(re[n]), vi = T(im[n]);
-    if (vr == T(0) && vi == T(0)) continue;
-    hbm_add<T>(g + st.grid_index(ctx, n), vr, vi);
-  }
+  tile_to_grid<DIM, BIN_THREADS, false>(ctx, st.P, st.off, st.ntile, re, im,
+                                        fw + int64_t(t) * nftot);
 }
 
 
-// ------------------------------------------------------- column-window (3-D)
+// ------------------------------------------------------- column spreading (3-D)
 //
 // The per-point LDS scatter above costs ns^3 LDS atomics per point. Here a
-// wave owns the 8 x 8 (x, y) grid columns of one sub-column (the points whose
-// first node lies in an S x S square of the bin, S = col_side(ns)), one
-// column per lane, and walks the sub-column's points in increasing z of their
-// first node. Each lane keeps the ns z-planes of its column that the current
-// point can touch in registers (register k holds plane zb + k), adds
-// c * wx * wy * wz[k] to them, and hands plane zb to the LDS tile (one fp64
-// atomic add per lane) only when the walk moves past it, shifting the window
-// up one plane. Per point the wave does 2 ns FMAs per lane and reads its
-// staged kernel values from LDS; the LDS atomics drop from ns^3 per point to
-// about 64 per plane of a sub-column.
+// wave owns the FOOT x FOOT (x, y) grid columns of one sub-column (the points
+// whose first node lies in an S x S square of the bin, S = col_side(ns),
+// FOOT = S + ns - 1 <= 8), one column per lane, and holds the column's whole
+// depth in the tile (bin depth + ns - 1 planes, at most COL_ZC + ns - 1) in
+// registers. A point with first z plane lz adds c wx wy wz[m] to planes
+// lz + m, m < ns, of every lane's column: 2 ns FMAs per lane. The sort orders
+// a sub-column's points by lz, so the walk visits the planes in a pass
+// unrolled at compile time and meets each run of equal lz with its register
+// indices fixed (no branch on lz, every accumulator of a run's loop updated
+// in place). When the walk passes into another sub-column the touched planes
+// of the columns are added to the LDS tile (one fp64 atomic add per lane,
+// plane and part) and cleared. The tile is fp64 because this GPU's fp64 LDS
+// atomic add runs about 20 times faster than its fp32 one.
 //
-// Staging: lane k of the wave evaluates the kernel of point p + k and writes
-// one record of COL_REC floats: wx[8] and wy[8] (the x and y weights already
-// shifted to the footprint column of each lane, zero outside), wz[8], c; the
-// point's first z plane stays in that lane's register and is read with
-// readlane.
+// A workgroup's points (a bin, or a subproblem of a dense bin) are dealt to
+// its waves as equal contiguous ranges of the sorted order, in which the
+// points of a sub-column are contiguous; a sub-column split between two waves
+// is added to the tile by both. A wave stages 64 points at a time: lane k
+// evaluates the kernel of point p + k and writes a record of COL_REC floats:
+// wz (8 floats, zero past ns: two broadcast 128-bit reads), then c wx as
+// (re, im) pairs shifted to the lanes' columns (16 floats, one 64-bit read
+// per lane) and wy shifted likewise (8 floats, one 32-bit read), all zero
+// outside the support. The point's first z plane and sub-column stay in
+// that lane's registers and are read with readlane.
 
-// Hands plane zb (register 0) to the tile cell and shifts the window up.
-template <int NS>
-__device__ __forceinline__ void col_shift(float (&ar)[NS], float (&ai)[NS], double *re,
-                                          double *im, int cell, bool on) {
-  if (on && (ar[0] != 0.f || ai[0] != 0.f)) {
-    atomicAdd(re + cell, double(ar[0]));
-    atomicAdd(im + cell, double(ai[0]));
-  }
-#pragma unroll
-  for (int k = 0; k + 1 < NS; ++k) {
-    ar[k] = ar[k + 1];
-    ai[k] = ai[k + 1];
-  }
-  ar[NS - 1] = 0.f;
-  ai[NS - 1] = 0.f;
-}
-
-// Hands all ns window planes zb .. zb + ns - 1 to the tile and clears them.
-template <int NS>
-__device__ __forceinline__ void col_flush_all(int zb, float (&ar)[NS], float (&ai)[NS],
-                                              double *re, double *im, int colcell, int plane,
-                                              bool on) {
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    if (on && (ar[k] != 0.f || ai[k] != 0.f)) {
-      const int cell = colcell + (zb + k) * plane;
-      atomicAdd(re + cell, double(ar[k]));
-      atomicAdd(im + cell, double(ai[k]));
-    }
-    ar[k] = 0.f;
-    ai[k] = 0.f;
-  }
-}
-
-// Staged values of one point as one lane of the wave sees them.
-template <int NS>
-struct ColPoint {
-  float wx, wy, cr, ci, wz[NS];
-  __device__ __forceinline__ void load(const float *stage, int k, int li, int lj) {
-    const float *r = stage + k * COL_REC;
-    wx = r[li];
-    wy = r[8 + lj];
-    cr = r[24];
-    ci = r[25];
-#pragma unroll
-    for (int m = 0; m < NS; ++m) wz[m] = r[16 + m];
+// One point's staged values as one lane sees them.
+struct ColRec {
+  float4 z0, z1;  // wz[0..7]
+  float2 xc;      // c wx[li]
+  float y;        // wy[lj]
+  __device__ __forceinline__ void load(const float *stage, int k, int xoff, int yoff) {
+    const float *rk = stage + k * COL_REC;
+    z0 = reinterpret_cast<const float4 *>(rk)[0];
+    z1 = reinterpret_cast<const float4 *>(rk)[1];
+    xc = *reinterpret_cast<const float2 *>(rk + xoff);
+    y = rk[yoff];
   }
 };
 
-// Moves the window up to plane lz (handing the passed planes to the tile) and
-// adds the point's contributions to the window.
-template <int NS>
-__device__ __forceinline__ void col_accumulate(const ColPoint<NS> &pt, int lz, int &zb,
-                                               float (&ar)[NS], float (&ai)[NS], double *re,
-                                               double *im, int colcell, bool on) {
-  constexpr int PLANE = (8 + NS - 1) * (8 + NS - 1);
-  if (lz > zb) {
-    if (lz - zb >= NS) {
-      col_flush_all<NS>(zb, ar, ai, re, im, colcell, PLANE, on);
-      zb = lz;
-    } else {
+// Adds the run of points k, k + 1, ... of sub-column position `base`
+// (lz + 64 * sub-column) whose first plane is Z, then the runs of planes
+// Z + 1 .. COL_ZC - 1.
+template <int NS, int ZD, int Z>
+__device__ __forceinline__ void col_walk(int &k, int n, int base, int posv, const float *stage,
+                                         int xoff, int yoff, float (&ar)[ZD], float (&ai)[ZD],
+                                         int &zlo, int &zhi) {
+  if constexpr (Z < COL_ZC) {
+    if (k < n && __builtin_amdgcn_readlane(posv, k) == base + Z) {
+      zlo = min(zlo, Z);
+      zhi = max(zhi, Z + NS);
       do {
-        col_shift<NS>(ar, ai, re, im, colcell + zb * PLANE, on);
-        ++zb;
-      } while (zb < lz);
+        ColRec rec;
+        rec.load(stage, k, xoff, yoff);
+        const float wr = rec.xc.x * rec.y, wi = rec.xc.y * rec.y;
+        const float wz[8] = {rec.z0.x, rec.z0.y, rec.z0.z, rec.z0.w,
+                             rec.z1.x, rec.z1.y, rec.z1.z, rec.z1.w};
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+          ar[Z + m] = std::fma(wr, wz[m], ar[Z + m]);
+          ai[Z + m] = std::fma(wi, wz[m], ai[Z + m]);
+        }
+        ++k;
+      } while (k < n && __builtin_amdgcn_readlane(posv, k) == base + Z);
+    }
+    col_walk<NS, ZD, Z + 1>(k, n, base, posv, stage, xoff, yoff, ar, ai, zlo, zhi);
+  }
+}
+
+// Adds planes [z0, z1) of the register columns to the tile at column cell
+// `colcell` (lanes with `on`), then clears all planes (the untouched ones
+// are zero already; clearing them too keeps every register write
+// unconditional, which the compiler keeps in place). z0, z1 are
+// wave-uniform; the plane stride PLANE is a compile-time constant, so each
+// add uses the LDS instruction's address offset.
+template <int ZD, int PLANE>
+__device__ __forceinline__ void col_flush(float (&ar)[ZD], float (&ai)[ZD], double *re,
+                                          double *im, int colcell, int z0, int z1, bool on) {
+  double *cr = re + colcell, *ci = im + colcell;
+#pragma unroll
+  for (int p = 0; p < ZD; ++p) {
+    if (p >= z0 && p < z1 && on) {
+      atomicAdd(cr + p * PLANE, double(ar[p]));
+      atomicAdd(ci + p * PLANE, double(ai[p]));
     }
   }
-  const float w = pt.wx * pt.wy;
-  const float wr = w * pt.cr, wi = w * pt.ci;
 #pragma unroll
-  for (int m = 0; m < NS; ++m) {
-    ar[m] = std::fma(wr, pt.wz[m], ar[m]);
-    ai[m] = std::fma(wi, pt.wz[m], ai[m]);
-  }
+  for (int p = 0; p < ZD; ++p) ar[p] = ai[p] = 0.f;
 }
 
-// Tile side in x and y of the column spreader (bins of 8 first nodes).
-template <int NS>
-constexpr int col_tile_xy() {
-  return 8 + NS - 1;
-}
-
-// Fine-grid index of column-tile cell n (x fastest; compile-time x, y sides).
-template <int NS>
-__device__ __forceinline__ int64_t col_grid_index(const SpreadCtx<float> &ctx,
-                                                  const StartTile<3, NS> &st, int n) {
-  constexpr int P = col_tile_xy<NS>();
-  const int x = n % P, t = n / P;
-  const int y = t % P, z = t / P;
-  const int gx = wrap_grid(x + st.off[0], ctx.nf[0]);
-  const int gy = wrap_grid(y + st.off[1], ctx.nf[1]);
-  const int gz = wrap_grid(z + st.off[2], ctx.nf[2]);
-  return (int64_t(gz) * ctx.nf[1] + gy) * ctx.nf[0] + gx;
-}
-
-// Accumulates the unit's points into the fp64 LDS tile (re, im planes).
-//
-// A wave walks its sub-columns (wave, wave + COL_WAVES, ...) in batches of up
-// to 64 points. The sorted index and coordinates of the next batch are loaded
-// into registers while the current batch is processed, and the strength
-// gather of a batch is issued before its kernel evaluation, so that global
-// memory latency overlaps compute.
-template <int NS>
-__device__ __forceinline__ void col_spread_into_tile(const SpreadCtx<float> &ctx,
-                                                     const StartTile<3, NS> &st,
-                                                     const float2 *__restrict__ c, double *re,
-                                                     double *im, float *stage_all) {
+// Accumulates points [u0, u1) of the unit into the fp64 LDS tile (re, im
+// planes of P x P x P2 cells, x fastest, P = 8 + ns - 1).
+template <int NS, typename Start>
+__device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
+                                                 const StartTile<3, NS> &st,
+                                                 const float2 *__restrict__ c, double *re,
+                                                 double *im, float *stage, int u0, int u1,
+                                                 Start &&start) {
   constexpr int S = col_side(NS);
-  constexpr int FOOT = S + NS - 1;  // footprint columns per dimension (<= 8)
-  constexpr int P = col_tile_xy<NS>();
-  constexpr int NSX = 8 / S;        // sub-columns per bin side
-  constexpr int NSUB = NSX * NSX;
-  constexpr int PLANE = P * P;
-  // Loop bounds and the window position are wave-uniform; readfirstlane
-  // keeps them in scalar registers so that the control flow stays scalar.
+  constexpr int FOOT = S + NS - 1;
+  constexpr int ZD = COL_ZC + NS - 1;  // register planes per column
+  constexpr int P = 8 + NS - 1;        // tile side in x and y (bins of 8)
+  constexpr int NSX = 8 / S;           // sub-columns per bin side
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 7, lj = lane >> 3;
   const bool on = li < FOOT && lj < FOOT;
-  float *stage = stage_all + wave * (64 * COL_REC);
-  const int u0 = __builtin_amdgcn_readfirstlane(st.first);
-  const int u1 = u0 + __builtin_amdgcn_readfirstlane(st.npts);
-  const int *cs = ctx.colstart + int64_t(__builtin_amdgcn_readfirstlane(st.bin)) * NSUB;
+  const int xoff = 8 + 2 * li, yoff = 24 + lj;
 
-  // First non-empty sub-column at or after sub; its point range in [a, b).
-  auto find_sub = [&](int sub, int &a, int &b) {
-    for (; sub < NSUB; sub += COL_WAVES) {
-      a = __builtin_amdgcn_readfirstlane(max(cs[sub], u0));
-      b = __builtin_amdgcn_readfirstlane(min(cs[sub + 1], u1));
-      if (a < b) break;
-    }
-    return sub;
-  };
   // Sorted index and coordinates of one lane's point of the next batch,
   // loaded while the current batch is processed.
   int fj = 0;
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  auto fetch = [&](int q, int end) {
-    if (q < end) {
+  auto fetch = [&](int q) {
+    if (q < u1) {
       fj = ctx.idx[q];
       fx = coord(ctx, 0, q, fj);
       fy = coord(ctx, 1, q, fj);
       fz = coord(ctx, 2, q, fj);
     }
   };
+  // The first batch's loads are issued before the workgroup's start-up work
+  // (clearing the tile), and each next batch's strengths while the current
+  // batch is walked, so that memory latency overlaps work.
+  fetch(u0 + lane);
+  float2 fc = make_float2(0.f, 0.f);
+  if (u0 + lane < u1) fc = c[fj];
+  start();
 
-  int p = 0, pe = 0;
-  int sub = find_sub(wave, p, pe);
-  if (sub < NSUB) fetch(p + lane, pe);
-  float ar[NS], ai[NS];
+  float ar[ZD], ai[ZD];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) ar[k] = ai[k] = 0.f;
-  int zb = 0;  // plane held in register 0
+  for (int k = 0; k < ZD; ++k) ar[k] = ai[k] = 0.f;
+  int cur = -1;           // sub-column held in the registers
+  int zlo = ZD, zhi = 0;  // planes [zlo, zhi) touched since the last flush
+  int colcell = 0;
 
-  while (sub < NSUB) {
-    const int n = min(64, pe - p);
-    const int j = fj;
+  for (int p = u0; p < u1; p += 64) {
+    const int n = min(64, u1 - p);
     const float x = fx, y = fy, z = fz;
-    // next batch: the rest of this sub-column or the next non-empty one
-    int np = p + 64, npe = pe, nsub = sub;
-    if (np >= pe) nsub = find_sub(sub + COL_WAVES, np, npe);
-    if (nsub < NSUB) fetch(np + lane, npe);
-
-    const int ox = (sub % NSX) * S, oy = (sub / NSX) * S;
-    // Tile cell of this lane's column at z = 0 (only used when `on`).
-    const int colcell = on ? (oy + lj) * P + ox + li : 0;
-    int lzv = 0;  // lane k: first z plane of point p + k
+    const float2 cv = fc;
+    fetch(p + 64 + lane);
+    int posv = 0;  // lz + 64 * sub-column of this lane's point
     if (lane < n) {
-      const float2 cv = c[j];  // issued before the kernel evaluation
       float kx[NS], ky[NS], kz[NS];
       const int s0 = start_and_kernel<float, NS>(x, ctx.nf[0], kx, ctx.kp);
       const int s1 = start_and_kernel<float, NS>(y, ctx.nf[1], ky, ctx.kp);
       const int s2 = start_and_kernel<float, NS>(z, ctx.nf[2], kz, ctx.kp);
-      const int lx = min(max(st.local_start(s0, 0, ctx.nf[0]) - ox, 0), S - 1);
-      const int ly = min(max(st.local_start(s1, 1, ctx.nf[1]) - oy, 0), S - 1);
-      lzv = st.local_start(s2, 2, ctx.nf[2]);
-      float *r = stage + lane * COL_REC;
+      const int lxl = st.local_start(s0, 0, ctx.nf[0]);
+      const int lyl = st.local_start(s1, 1, ctx.nf[1]);
+      // < bin depth <= COL_ZC (the plan's bins); the clamp keeps every
+      // point inside the planes the walk visits, so that it always advances
+      const int lz = min(st.local_start(s2, 2, ctx.nf[2]), COL_ZC - 1);
+      const int sx = lxl / S, sy = lyl / S;
+      posv = lz + 64 * (sx + NSX * sy);
+      const int lx = lxl - sx * S, ly = lyl - sy * S;
+      // x and y weights shifted to the lanes' columns: column t holds
+      // k[t - lx] (lx < S), zero outside the support.
+      float wx8[8], wy8[8];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) r[t] = 0.f;
+      for (int t = 0; t < 8; ++t) {
+        float vx = 0.f, vy = 0.f;
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        r[lx + k] = kx[k];
-        r[8 + ly + k] = ky[k];
-        r[16 + k] = kz[k];
+        for (int o = 0; o < S; ++o) {
+          if (t - o >= 0 && t - o < NS) {
+            vx = lx == o ? kx[t - o] : vx;
+            vy = ly == o ? ky[t - o] : vy;
+          }
+        }
+        wx8[t] = vx;
+        wy8[t] = vy;
       }
-      r[24] = cv.x;
-      r[25] = cv.y;
+      float wz8[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) wz8[m] = m < NS ? kz[m] : 0.f;
+      float4 *r = reinterpret_cast<float4 *>(stage + lane * COL_REC);
+      r[0] = make_float4(wz8[0], wz8[1], wz8[2], wz8[3]);
+      r[1] = make_float4(wz8[4], wz8[5], wz8[6], wz8[7]);
+#pragma unroll
+      for (int t = 0; t < 8; t += 2)
+        r[2 + t / 2] = make_float4(cv.x * wx8[t], cv.y * wx8[t], cv.x * wx8[t + 1],
+                                   cv.y * wx8[t + 1]);
+      r[6] = make_float4(wy8[0], wy8[1], wy8[2], wy8[3]);
+      r[7] = make_float4(wy8[4], wy8[5], wy8[6], wy8[7]);
     }
+    if (p + 64 + lane < u1) fc = c[fj];  // the next batch's strengths
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 
-    // Two-stage pipeline: the LDS reads of point k + 1 are issued before
-    // point k is accumulated (records past n are in bounds and unused).
-    ColPoint<NS> cur, nxt;
-    cur.load(stage, 0, li, lj);
-    for (int k = 0; k < n; k += 2) {
-      nxt.load(stage, min(k + 1, 63), li, lj);
-      col_accumulate<NS>(cur, __builtin_amdgcn_readlane(lzv, k), zb, ar, ai, re, im, colcell,
-                         on);
-      if (k + 1 >= n) break;
-      cur.load(stage, min(k + 2, 63), li, lj);
-      col_accumulate<NS>(nxt, __builtin_amdgcn_readlane(lzv, k + 1), zb, ar, ai, re, im,
-                         colcell, on);
+    int k = 0;
+    while (k < n) {
+      const int sc = __builtin_amdgcn_readlane(posv, k) >> 6;
+      if (sc != cur) {  // another sub-column: hand the columns to the tile
+        if (cur >= 0) col_flush<ZD, P * P>(ar, ai, re, im, colcell, zlo, zhi, on);
+        cur = sc;
+        zlo = ZD;
+        zhi = 0;
+        colcell = ((sc / NSX) * S + lj) * P + (sc % NSX) * S + li;
+      }
+      col_walk<NS, ZD, 0>(k, n, 64 * sc, posv, stage, xoff, yoff, ar, ai, zlo, zhi);
     }
     // all lanes are done reading this batch's records before restaging
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-
-    if (nsub != sub) {  // sub-column done: hand the window to the tile
-      col_flush_all<NS>(zb, ar, ai, re, im, colcell, PLANE, on);
-      zb = 0;
-    }
-    sub = nsub;
-    p = np;
-    pe = npe;
   }
+  if (cur >= 0) col_flush<ZD, P * P>(ar, ai, re, im, colcell, zlo, zhi, on);
+}
+
+// The unit's tile (zeroed) and its points dealt to the waves.
+template <int NS>
+__device__ __forceinline__ void col_spread_unit(const SpreadCtx<float> &ctx,
+                                                const StartTile<3, NS> &st,
+                                                const float2 *__restrict__ c, double *re,
+                                                double *im, float *stage_all) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int first = __builtin_amdgcn_readfirstlane(st.first);
+  const int npts = __builtin_amdgcn_readfirstlane(st.npts);
+  const int u0 = first + int(int64_t(npts) * wave / COL_WAVES);
+  const int u1 = first + int(int64_t(npts) * (wave + 1) / COL_WAVES);
+  // every wave runs the range (an empty one too): the tile is cleared and
+  // the workgroup synchronized inside, after the first loads are issued
+  col_spread_range<NS>(ctx, st, c, re, im, stage_all + wave * (64 * COL_REC), u0, u1, [&] {
+    for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
+      re[n] = 0.0;
+      im[n] = 0.0;
+    }
+    __syncthreads();
+  });
+  __syncthreads();
 }
 
 // Whole-bin column spreading in parity colors with a plain tile flush; bins
@@ -639,28 +730,16 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
                               float2 *__restrict__ fw, int64_t nftot, int col0, int col1,
                               int col2) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  double *re = reinterpret_cast<double *>(lds_raw);
   StartTile<3, NS> st;
   st.init_bin(ctx, colored_bin<3>(ctx, blockIdx.x, col0, col1, col2));
   if (st.npts == 0 || st.npts > ctx.dense_min) return;
+  double *re = reinterpret_cast<double *>(lds_raw);
   double *im = re + st.ntile;
-  float *stage = reinterpret_cast<float *>(im + st.ntile);
   const int t = blockIdx.y;
-  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
-    re[n] = 0.0;
-    im[n] = 0.0;
-  }
-  __syncthreads();
-  col_spread_into_tile<NS>(ctx, st, c + int64_t(t) * ctx.M, re, im, stage);
-  __syncthreads();
-  float2 *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
-    const int64_t gi = col_grid_index<NS>(ctx, st, n);
-    float2 v = g[gi];
-    v.x += float(re[n]);
-    v.y += float(im[n]);
-    g[gi] = v;
-  }
+  col_spread_unit<NS>(ctx, st, c + int64_t(t) * ctx.M, re, im,
+                      reinterpret_cast<float *>(im + st.ntile));
+  tile_to_grid<3, 64 * COL_WAVES, true>(ctx, st.P, st.off, st.ntile, re, im,
+                                        fw + int64_t(t) * nftot);
 }
 
 // Column spreading of the dense bins' subproblems with an atomic tile flush.
@@ -669,25 +748,15 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
     spread_col_sub_kernel(SpreadCtx<float> ctx, const float2 *__restrict__ c,
                           float2 *__restrict__ fw, int64_t nftot) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  double *re = reinterpret_cast<double *>(lds_raw);
   StartTile<3, NS> st;
   if (!st.init_sub(ctx, blockIdx.x)) return;
+  double *re = reinterpret_cast<double *>(lds_raw);
   double *im = re + st.ntile;
-  float *stage = reinterpret_cast<float *>(im + st.ntile);
   const int t = blockIdx.y;
-  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
-    re[n] = 0.0;
-    im[n] = 0.0;
-  }
-  __syncthreads();
-  col_spread_into_tile<NS>(ctx, st, c + int64_t(t) * ctx.M, re, im, stage);
-  __syncthreads();
-  float2 *g = fw + int64_t(t) * nftot;
-  for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
-    const float vr = float(re[n]), vi = float(im[n]);
-    if (vr == 0.f && vi == 0.f) continue;
-    hbm_add<float>(g + col_grid_index<NS>(ctx, st, n), vr, vi);
-  }
+  col_spread_unit<NS>(ctx, st, c + int64_t(t) * ctx.M, re, im,
+                      reinterpret_cast<float *>(im + st.ntile));
+  tile_to_grid<3, 64 * COL_WAVES, false>(ctx, st.P, st.off, st.ntile, re, im,
+                                         fw + int64_t(t) * nftot);
 }
 
 // Workgroup size of the multi-vector interpolation, points per thread per
@@ -698,52 +767,6 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
 constexpr int MV_THREADS = 512;
 constexpr int MV_PT = 2;
 constexpr int MV_PF = 6;
-
-// Walks a tile in steps of MV_THREADS cells: each thread's first cell and
-// the step in tile coordinates are found once per unit, so that the
-// per-vector loads need no division.
-template <int DIM>
-struct TileWalk {
-  int x0, y0, z0, sx, sy, sz;
-  __device__ void init(const int *P) {
-    int r = threadIdx.x;
-    x0 = r % P[0];
-    r /= P[0];
-    y0 = DIM > 1 ? r % P[1] : 0;
-    z0 = DIM > 1 ? r / P[1] : 0;
-    r = MV_THREADS;
-    sx = r % P[0];
-    r /= P[0];
-    sy = DIM > 1 ? r % P[1] : 0;
-    sz = DIM > 1 ? r / P[1] : 0;
-  }
-  // Grid index of tile cell (x, y, z) of a tile at off (< 2^31: the plan
-  // caps a vector's grid there), then the next cell of the walk.
-  template <typename T>
-  __device__ __forceinline__ int index_and_step(const SpreadCtx<T> &ctx, const int *P,
-                                                const int *off, int &x, int &y, int &z) const {
-    int gi = wrap_grid(x + off[0], ctx.nf[0]);
-    if (DIM > 1) {
-      int row = wrap_grid(y + off[1], ctx.nf[1]);
-      if (DIM > 2) row += wrap_grid(z + off[2], ctx.nf[2]) * ctx.nf[1];
-      gi += row * ctx.nf[0];
-    }
-    x += sx;
-    if (x >= P[0]) {
-      x -= P[0];
-      ++y;
-    }
-    if (DIM > 1) {
-      y += sy;
-      if (y >= P[1]) {
-        y -= P[1];
-        ++z;
-      }
-      z += sz;
-    }
-    return gi;
-  }
-};
 
 // Interpolation of one subproblem over nvec vectors (the fused launch over
 // many problems). The points are taken in chunks of up to MV_PT per thread;
@@ -764,7 +787,7 @@ __device__ __forceinline__ void interp_unit_mv(const SpreadCtx<T> &ctx, const Ke
   gptr<const T> sx[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) sx[d] = to_global(ctx.sxyz[d]);
-  TileWalk<DIM> walk;
+  TileWalk<DIM, MV_THREADS> walk;
   walk.init(st.P);
   const bool pipelined = st.ntile <= MV_PF * MV_THREADS;
   for (int c0i = 0; c0i < st.npts; c0i += CH) {
@@ -887,7 +910,7 @@ template <typename T, int DIM, int NS>
 __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const StartTile<DIM, NS> &st,
                                             cplx_t<T> *tile, const cplx_t<T> *__restrict__ g,
                                             cplx_t<T> *__restrict__ cout) {
-  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) tile[n] = g[st.grid_index(ctx, n)];
+  grid_to_tile<DIM, BIN_THREADS>(ctx, st.P, st.off, st.ntile, g, tile);
   __syncthreads();
   // Strided dealing: work item i = 32 g + m takes the unit's point m G + g
   // (G = ceil(npts / 32)). When the bin is ordered by LDS bank class (the

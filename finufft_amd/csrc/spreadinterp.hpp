@@ -94,16 +94,23 @@ __device__ __forceinline__ kptr<P> to_const(const P *p) {
   return (kptr<P>)p;
 }
 
-// Column-window spreading geometry. A wave owns the footprint of one
-// sub-column: 8 x 8 (x, y) grid columns, one per lane, for the points whose
-// first node lies in an S x S square, S the largest power of two with
-// S + ns - 1 <= 8.
+// Column spreading geometry. A wave owns the footprint of one sub-column:
+// 8 x 8 (x, y) grid columns, one per lane, for the points whose first node
+// lies in an S x S square, S the largest power of two with S + ns - 1 <= 8;
+// each lane holds its column's whole depth in the bin's tile (bin depth at
+// most COL_ZC) in registers. The tile is fp64; every wave stages 64 points of
+// COL_REC floats.
 constexpr int COL_MAX_NS = 8;
-constexpr int COL_WAVES = 4;    // waves per workgroup
-constexpr int COL_REC = 28;     // floats staged per point
+constexpr int COL_WAVES = 4;  // waves per workgroup
+constexpr int COL_REC = 32;   // floats staged per point
+constexpr int COL_ZC = 8;     // largest bin depth (first-node planes)
 inline constexpr int col_side(int ns) { return ns <= 5 ? 4 : (ns <= 7 ? 2 : 1); }
 inline constexpr size_t col_stage_bytes() {
   return size_t(COL_WAVES) * 64 * COL_REC * sizeof(float);
+}
+// LDS bytes of a column-spreading unit with a tile of `cells` cells.
+inline constexpr size_t col_lds_bytes(int64_t cells) {
+  return size_t(cells) * 2 * sizeof(double) + col_stage_bytes();
 }
 
 // c (M x blk, contiguous per vector) -> fw (nf_tot x blk). fw must be zeroed

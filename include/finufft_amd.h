@@ -136,17 +136,33 @@ FINUFFT_EXPORT int64_t finufft_amd_next235(int64_t n, int64_t f);
  * finufft::common::gaussquad (src/common/utils.cpp:18-58). */
 FINUFFT_EXPORT void finufft_amd_gaussquad(int n, double *x, double *w);
 
-/* Kernel width and ES beta chosen for tol at upsampfac (is_double selects the
- * precision the tolerance is held in); reference: setup_spreadinterp
- * (src/cuda/makeplan.cu:43-119). Returns an error code (7 or 8 as makeplan). */
+/* Kernel width, shape parameter and family (0 ES, 1 PSWF) that a plan of
+ * this dimension and type chooses for tol at upsampfac (is_double selects the
+ * precision the tolerance is held in). kerevalmeth 1 gives the PSWF kernel
+ * with the width law of the reference CPU library (theoretical_kernel_ns and
+ * set_kernel_shape_given_ns, src/common/kernel.cpp:60-150); 0 and 2 give the
+ * ES kernel with the reference GPU library's rule (setup_spreadinterp,
+ * src/cuda/makeplan.cu:43-119). Returns an error code (7, 8 or 24 as
+ * makeplan). */
 FINUFFT_EXPORT int finufft_amd_kernel_shape(double tol, int is_double, double upsampfac,
-                                            int kerevalmeth, int *ns, double *beta);
+                                            int kerevalmeth, int dim, int type, int *ns,
+                                            double *beta, int *formula);
 
-/* Horner coefficients (nc rows of ns values, row k = z^k) of the kernel for
+/* Horner coefficients (nc rows of ns values, row k = z^k) of the ES kernel for
  * sigma = 2 or 1.25; reference tables include/cufinufft/contrib/
  * ker_horner_allw_loop.inc and ker_lowupsampfac_horner_allw_loop.inc, made by
  * devel/gen_all_horner_C_code.m. Returns the number of rows written. */
 FINUFFT_EXPORT int finufft_amd_horner_coeffs(int ns, double upsampfac, double *out, int nc_max);
+
+/* Horner coefficients (same layout) of the PSWF kernel of shape parameter
+ * beta; reference: precompute_horner_coeffs (include/finufft/makeplan.hpp:
+ * 204-315). Returns the number of rows written. */
+FINUFFT_EXPORT int finufft_amd_horner_coeffs_pswf(int ns, double beta, double *out, int nc_max);
+
+/* psi_0^c(z) normalised to 1 at z = 0 (0 for |z| > 1) at n points; reference:
+ * finufft::common::PSWF0 (include/finufft_common/pswf.h:25-61,
+ * src/common/pswf.cpp). */
+FINUFFT_EXPORT void finufft_amd_pswf(double c, const double *z, int64_t n, double *out);
 
 /* The device fold of a coordinate into [0, n) with the GPU rounding rules,
  * evaluated on the host with the same code; reference: fold_rescale

@@ -2,11 +2,12 @@
 
 Spread / interpolate / deconvolve run in C with OpenMP
 (oracle/csrc/cpu_nufft.c); the fine-grid FFT is scipy.fft with the same
-thread count. Kernel width, beta, Horner coefficients and the fine grid size
-come from oracle.kernel (the reference's rules). The kernel's Fourier series
+thread count. Kernel width, shape, Horner coefficients (PSWF) and the fine grid
+size come from oracle.kernel (the reference's rules). The kernel's Fourier series
 uses numpy's Gauss-Legendre rule with the node count of src/cuda/fft.cu:185-202.
 
-Pinned by tests/test_oracle_cpu.py against the FP64 direct sum. Test
+Pinned by tests/test_abi_cpu.py::test_cpu_restatement_vs_direct against the FP64
+direct sum. Test
 infrastructure and CPU baseline only (see oracle/__init__.py).
 """
 
@@ -34,12 +35,14 @@ def _cfuncs():
     return lib
 
 
-def fourier_series_gl(ns, beta, nf):
-    """(-1)^k sum_n f_n 2 cos(2 pi k z_n / nf), k = 0..nf/2 (Gauss-Legendre)."""
+def fourier_series_gl(ns, beta, nf, formula="pswf"):
+    """(-1)^k sum_n f_n 2 cos(2 pi k z_n / nf), k = 0..nf/2 (Gauss-Legendre),
+    of the PSWF (default) or ES kernel."""
     q = int(2 + 1.5 * ns)
     x, w = np.polynomial.legendre.leggauss(2 * q)
     z = (ns / 2.0) * x[:q]
-    f = (ns / 2.0) * w[:q] * kernel.es(z, ns, beta)
+    phi = kernel.pswf(beta, 2.0 * z / ns) if formula == "pswf" else kernel.es(z, ns, beta)
+    f = (ns / 2.0) * w[:q] * phi
     k = np.arange(nf // 2 + 1)
     out = (f[None, :] * 2 * np.cos(2 * math.pi * np.outer(k, z) / nf)).sum(axis=1)
     return out * np.where(k % 2 == 1, -1.0, 1.0)
@@ -55,8 +58,11 @@ class CpuPlan:
         self.isign = isign if isign is not None else (-1 if nufft_type == 2 else 1)
         self.dim = len(n_modes)
         self.ms_c = list(n_modes)[::-1]
-        self.ns, self.beta = kernel.kernel_shape(eps, 2.0, is_double=False)
-        coef = kernel.horner_coeffs(self.ns, 2.0)
+        # the library's default kernel: PSWF with the width law for this
+        # dimension and type
+        self.ns, self.beta = kernel.kernel_shape_pswf(eps, 2.0, len(n_modes), nufft_type, False)
+        nc = self.ns + 2 + (self.ns <= 7) - (self.ns == 2)
+        coef = kernel.horner_coeffs_pswf(self.ns, self.beta, nc)
         self.nc = coef.shape[0]
         self.coef = np.ascontiguousarray(coef, np.float64)
         self.nf_c = [kernel.fine_grid_len(m, 2.0, self.ns) for m in self.ms_c]

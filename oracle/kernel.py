@@ -15,6 +15,17 @@
 * ``fourier_series``: the kernel's continuous Fourier transform at integer
   frequencies by adaptive scipy quadrature (independent of the library's
   Gauss-Legendre rule, src/cuda/fft.cu:29-49 and :185-202).
+* ``pswf``: the prolate spheroidal wave function psi_0^c normalised to 1 at
+  0, the reference CPU library's kernel (include/finufft_common/pswf.h,
+  src/common/kernel.cpp:41-57): lowest eigenvector of the prolate operator
+  in the orthonormal even Legendre basis, by LAPACK's tridiagonal eigensolver
+  (scipy.linalg.eigh_tridiagonal), summed with numpy's Legendre series.
+* ``kernel_shape_pswf``: width law and shape parameter of the reference CPU
+  plan (theoretical_kernel_ns and clamp_kernel_ns, src/common/kernel.cpp:60-96
+  and include/finufft_common/kernel.h; set_kernel_shape_given_ns kerformula
+  8, kernel.cpp:98-150).
+* ``horner_coeffs_pswf``: per-interval least-squares polynomial fit of the
+  PSWF at Chebyshev points.
 
 Test infrastructure only (see oracle/__init__.py).
 """
@@ -85,6 +96,48 @@ def horner_coeffs(ns, upsampfac):
     return C[: d + 1].real
 
 
+def pswf(c, z):
+    """psi_0^c(z) / psi_0^c(0), 0 for |z| > 1."""
+    from numpy.polynomial import legendre
+    from scipy.linalg import eigh_tridiagonal
+
+    n = int(c) + 40
+    k = np.arange(0, 2 * n, 2).astype(np.float64)
+    b = (k + 1) ** 2 / ((2 * k + 1) * (2 * k + 3))
+    b[1:] += k[1:] ** 2 / ((2 * k[1:] + 1) * (2 * k[1:] - 1))
+    d = k * (k + 1) + c * c * b
+    kk = k[:-1]
+    e = c * c * (kk + 1) * (kk + 2) / ((2 * kk + 3) * np.sqrt((2 * kk + 1) * (2 * kk + 5)))
+    _, v = eigh_tridiagonal(d, e, select="i", select_range=(0, 0))
+    coef = np.zeros(2 * n)
+    coef[0::2] = v[:, 0] * np.sqrt(k + 0.5)
+    z = np.asarray(z, np.float64)
+    val = legendre.legval(np.clip(z, -1.0, 1.0), coef) / legendre.legval(0.0, coef)
+    return np.where(np.abs(z) <= 1.0, val, 0.0)
+
+
+def kernel_shape_pswf(tol, upsampfac, dim, ttype, is_double):
+    """(ns, beta) of a Horner (kerevalmeth 1) plan: the PSWF kernel."""
+    eps = max(float(tol), float(np.finfo(np.float64 if is_double else np.float32).eps))
+    if not is_double:
+        eps = float(np.float32(eps))
+    tolfac = 0.18 * 1.4 ** (dim - 1) * (1.4 if ttype == 3 else 1.0)
+    ns = math.ceil(math.log(tolfac / eps) / (math.pi * math.sqrt(1.0 - 1.0 / upsampfac)) + 1.0)
+    ns = min(max(2, ns), 16 if is_double else 12)
+    if not is_double and upsampfac < 1.4:
+        ns = min(ns, 8)
+    return ns, math.pi * ns * (1.0 - 1.0 / (2.0 * upsampfac)) - 0.05
+
+
+def horner_coeffs_pswf(ns, beta, nc):
+    """(nc, ns) coefficients, row k multiplies z^k on each interval."""
+    t = np.cos(np.pi * (2 * np.arange(4 * nc) + 1) / (8 * nc))
+    V = t[:, None] ** np.arange(nc)[None, :]
+    R = np.stack([pswf(beta, -1 + (2 * i + 1) / ns + t / ns) for i in range(ns)], axis=1)
+    C, *_ = np.linalg.lstsq(V, R, rcond=None)
+    return C
+
+
 def horner_eval(coef, z):
     """ker[i] at local offset z in [-1, 1] for each interval i."""
     out = np.zeros(coef.shape[1])
@@ -112,15 +165,21 @@ def fine_grid_len(ms, upsampfac, ns):
     return next235(nf, 2)
 
 
-def fourier_series(ns, beta, nf, kmax):
+def fourier_series(ns, beta, nf, kmax, formula="es"):
     """phihat(k) = int phi(x) exp(-2 pi i k x / nf) dx over |x| < ns/2 (grid units),
-    times (-1)^k, for k = 0..kmax."""
+    times (-1)^k, for k = 0..kmax; phi the ES or the PSWF kernel."""
     from scipy.integrate import quad
 
+    if formula == "es":
+        def phi(x):
+            return float(es(x, ns, beta))
+    else:
+        def phi(x):
+            return float(pswf(beta, 2.0 * x / ns))
     out = np.zeros(kmax + 1)
     for k in range(kmax + 1):
         w = 2 * math.pi * k / nf
-        val, _ = quad(lambda x: float(es(x, ns, beta)) * math.cos(w * x), -ns / 2, ns / 2,
+        val, _ = quad(lambda x: phi(x) * math.cos(w * x), -ns / 2, ns / 2,
                       limit=200, epsabs=1e-14, epsrel=1e-13)
         out[k] = val * (-1) ** k
     return out

@@ -270,11 +270,11 @@ def test_interp_grad_kernel_vs_direct(tmp_path_factory, tmp_path, geom):
 
 def test_column_spreader_spread_only_grid(tmp_path_factory, tmp_path):
     """The geometry of a spread-only 3-D fp32 plan on a 48 x 40 x 44 mode grid
-    (nf = the mode counts, x fastest: 44 x 40 x 48): bins of 8 x 8 x 10 whose
+    (nf = the mode counts, x fastest: 44 x 40 x 48): bins of 8 x 8 x 8 whose
     x count does not divide the grid, so every bin runs as subproblems of the
     column spreader, on clustered points."""
     exe = _exe(tmp_path_factory, "float", 3, 7)
-    nf, bs = (44, 40, 48), (8, 8, 10)
+    nf, bs = (44, 40, 48), (8, 8, 8)
     nsub, xs, c, fw = _run(exe, tmp_path, "colsub", nf, bs, 2000, 1024, 1, 1, 16)
     idx, w = _weights(xs, nf, 7, 2.30 * 7)
     ref = np.zeros(int(np.prod(nf)), complex)

@@ -100,10 +100,12 @@ def test_upsampfac_and_direct_kernel(torch_gpu, upsampfac, kerevalmeth):
         assert err < 25 * tol
 
 
-@pytest.mark.parametrize("kernel_eps,bound", [(1e-6, 2e-6), (1e-7, 1e-6)])
-def test_fp32_3d_tol_1e6(torch_gpu, kernel_eps, bound):
-    """3-D fp32 at tol 1e-6. The reference width rule (ns = 7 for tol 1e-6)
-    lands at ~1.4e-6; a kernel chosen for 1e-7 (ns = 8) meets 1e-6."""
+@pytest.mark.parametrize("kerevalmeth,kernel_eps,bound", [(1, 1e-6, 1e-6), (2, 1e-6, 2e-6),
+                                                          (2, 1e-7, 1e-6)])
+def test_fp32_3d_tol_1e6(torch_gpu, kerevalmeth, kernel_eps, bound):
+    """3-D fp32 at tol 1e-6. The PSWF kernel (kerevalmeth 1, width 7) meets
+    1e-6; the reference GPU library's ES kernel at its width 7 (kerevalmeth
+    2) lands at ~1.4e-6, and meets 1e-6 only at width 8 (tol 1e-7)."""
     for ttype in (1, 2):
         for method in (1, 2):
             from finufft_amd import Plan
@@ -113,8 +115,10 @@ def test_fp32_3d_tol_1e6(torch_gpu, kernel_eps, bound):
             shape = (32, 32, 32)
             M = 20000
             pts = uniform_pts(r, 3, M, np.float32)
-            plan = Plan(ttype, shape, eps=kernel_eps, dtype="complex64", gpu_method=method)
+            plan = Plan(ttype, shape, eps=kernel_eps, dtype="complex64", gpu_method=method,
+                        gpu_kerevalmeth=kerevalmeth)
             plan.setpts(*[to_dev(torch, p) for p in pts])
+            assert plan.info()["ns"] == (7 if kernel_eps == 1e-6 else 8)
             p64 = [p.astype(np.float64) for p in pts]
             if ttype == 1:
                 c = cnormal(r, (M,), np.complex64)
@@ -238,12 +242,12 @@ def _type1_case(torch, shape, tol, pts, c, **kw):
     return nudft.relerr(f, ref), plan.info()
 
 
-@pytest.mark.parametrize("tol,ns", [(1e-3, 4), (2e-4, 5), (1e-4, 6), (1e-6, 7), (1e-7, 8)])
+@pytest.mark.parametrize("tol,ns", [(1e-3, 4), (2e-4, 5), (2e-5, 6), (1e-6, 7), (1e-7, 8)])
 def test_column_spread_widths(torch_gpu, tol, ns):
     """3-D fp32 column-window spreading at kernel widths 4..8 (sub-column
     sides 4, 4, 2, 2, 1) against the direct sum, 25 tol (1e-6 absolute floor
-    where fp32 rounding dominates). The widths are those of the fp32 width
-    rule (finufft_amd_kernel_shape)."""
+    where fp32 rounding dominates). The widths are those of the PSWF width
+    law for 3-D type 1 in fp32 (finufft_amd_kernel_shape)."""
     side = 4 if ns <= 5 else (2 if ns <= 7 else 1)
     r = rng(21)
     shape = (16, 24, 32)

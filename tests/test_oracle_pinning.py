@@ -114,22 +114,56 @@ def test_gaussquad():
 # ---------------------------------------------------------------- width rules
 
 
-@pytest.mark.parametrize("is_double", [0, 1])
-@pytest.mark.parametrize("sigma,kerevalmeth", [(2.0, 1), (1.25, 1), (1.5, 0), (3.0, 0)])
-def test_kernel_shape_library_vs_oracle(is_double, sigma, kerevalmeth):
+def _lib_kernel_shape(tol, is_double, sigma, kerevalmeth, dim=3, ttype=1):
     import ctypes
 
     from finufft_amd import _lib
 
-    L = _lib.lib()
-    for tol in [1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-12, 1e-15, 1e-20]:
-        ns = ctypes.c_int()
-        beta = ctypes.c_double()
-        ier = L.finufft_amd_kernel_shape(tol, is_double, sigma, kerevalmeth, ctypes.byref(ns),
-                                         ctypes.byref(beta))
-        assert ier == 0
+    ns, beta, formula = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    ier = _lib.lib().finufft_amd_kernel_shape(tol, is_double, sigma, kerevalmeth, dim, ttype,
+                                               ctypes.byref(ns), ctypes.byref(beta),
+                                               ctypes.byref(formula))
+    return ier, ns.value, beta.value, formula.value
+
+
+TOLS = [1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-12, 1e-15, 1e-20]
+
+
+@pytest.mark.parametrize("is_double", [0, 1])
+@pytest.mark.parametrize("sigma,kerevalmeth", [(2.0, 2), (1.25, 2), (2.0, 0), (1.5, 0), (3.0, 0)])
+def test_kernel_shape_library_vs_oracle(is_double, sigma, kerevalmeth):
+    """ES plans (kerevalmeth 0 and 2): the reference GPU library's rule."""
+    for tol in TOLS:
+        ier, ns, beta, formula = _lib_kernel_shape(tol, is_double, sigma, kerevalmeth)
+        assert ier == 0 and formula == 0
         ons, obeta = kernel.kernel_shape(tol, sigma, bool(is_double))
-        assert (ns.value, beta.value) == pytest.approx((ons, obeta)), tol
+        assert (ns, beta) == pytest.approx((ons, obeta)), tol
+
+
+@pytest.mark.parametrize("is_double", [0, 1])
+@pytest.mark.parametrize("sigma", [2.0, 1.25])
+def test_kernel_shape_pswf_library_vs_oracle(is_double, sigma):
+    """Horner plans (kerevalmeth 1): the PSWF kernel with the reference CPU
+    library's width law, for every dimension and type."""
+    for dim in (1, 2, 3):
+        for ttype in (1, 2, 3):
+            for tol in TOLS:
+                ier, ns, beta, formula = _lib_kernel_shape(tol, is_double, sigma, 1, dim, ttype)
+                assert ier == 0 and formula == 1
+                ons, obeta = kernel.kernel_shape_pswf(tol, sigma, dim, ttype, bool(is_double))
+                assert (ns, beta) == pytest.approx((ons, obeta), abs=1e-12), (dim, ttype, tol)
+
+
+def test_kernel_shape_pswf_known_values():
+    """The headline configuration (3-D, fp32, tol 1e-6) keeps width 7 with the
+    PSWF kernel, c = 7 pi 3/4 - 0.05; 2-D fp32 tol 1e-5 gives 6 and 1-D fp64
+    tol 1e-9 gives 10, the widths of the ES rule at those configurations."""
+    assert kernel.kernel_shape_pswf(1e-6, 2.0, 3, 1, False) == pytest.approx(
+        (7, 7 * math.pi * 0.75 - 0.05))
+    assert kernel.kernel_shape_pswf(1e-5, 2.0, 2, 2, False)[0] == 6
+    assert kernel.kernel_shape_pswf(1e-9, 2.0, 1, 1, True)[0] == 10
+    # fp32 below sigma 1.4 caps the width at 8
+    assert kernel.kernel_shape_pswf(1e-7, 1.25, 3, 1, False)[0] == 8
 
 
 def test_kernel_shape_known_values():
@@ -141,15 +175,12 @@ def test_kernel_shape_known_values():
 
 
 def test_kernel_shape_errors():
-    import ctypes
-
-    from finufft_amd import _lib
-
-    L = _lib.lib()
-    ns = ctypes.c_int()
-    beta = ctypes.c_double()
-    assert L.finufft_amd_kernel_shape(1e-5, 0, 0.9, 1, ctypes.byref(ns), ctypes.byref(beta)) == 8
-    assert L.finufft_amd_kernel_shape(1e-5, 0, 0.9, 0, ctypes.byref(ns), ctypes.byref(beta)) == 7
+    """Horner with a nonstandard sigma -> 8, sigma <= 1 -> 7
+    (makeplan.cu:61-77); an unknown kernel evaluation method -> 24."""
+    assert _lib_kernel_shape(1e-5, 0, 0.9, 1)[0] == 8
+    assert _lib_kernel_shape(1e-5, 0, 0.9, 2)[0] == 8
+    assert _lib_kernel_shape(1e-5, 0, 0.9, 0)[0] == 7
+    assert _lib_kernel_shape(1e-5, 0, 2.0, 3)[0] == 24
 
 
 def test_fine_grid_len():
@@ -213,10 +244,14 @@ def test_fourier_series_gl_vs_quadrature():
 
     for ns, nf in ((7, 64), (10, 40), (4, 100)):
         beta = 2.30 * ns if ns > 4 else 2.38 * ns
-        gl = cpu.fourier_series_gl(ns, beta, nf)
-        q = kernel.fourier_series(ns, beta, nf, nf // 2)
+        gl = cpu.fourier_series_gl(ns, beta, nf, "es")
+        q = kernel.fourier_series(ns, beta, nf, nf // 2, "es")
         # the q = floor(2 + 1.5 ns) node rule is accurate far below the
         # width's own tolerance 10^-(ns-1)
+        assert np.abs(gl - q).max() < 10.0 ** (-(ns + 1)) * np.abs(q).max()
+        c = math.pi * ns * 0.75 - 0.05
+        gl = cpu.fourier_series_gl(ns, c, nf, "pswf")
+        q = kernel.fourier_series(ns, c, nf, nf // 2, "pswf")
         assert np.abs(gl - q).max() < 10.0 ** (-(ns + 1)) * np.abs(q).max()
 
 
@@ -339,19 +374,17 @@ def test_subcolumn_sort_vs_bruteforce():
 
 
 def test_fp32_width_table():
-    """fp32 kernel widths used by the 3-D column spreader tests (the reference
-    rule evaluated with tol / 10 in fp32, src/cuda/makeplan.cu:43-119)."""
-    import ctypes
-
-    from finufft_amd import _lib
-
-    lib = _lib.lib()
+    """fp32 kernel widths used by the 3-D column spreader tests: the PSWF law
+    for 3-D type 1 (kerevalmeth 1) and the ES rule (tol / 10 in fp32,
+    src/cuda/makeplan.cu:43-119, kerevalmeth 2)."""
+    for tol, want in [(1e-3, 4), (2e-4, 5), (2e-5, 6), (1e-6, 7), (1e-7, 8)]:
+        ier, ns, _, _ = _lib_kernel_shape(tol, 0, 2.0, 1, 3, 1)
+        assert ier == 0 and ns == want, tol
+        assert ns == kernel.kernel_shape_pswf(tol, 2.0, 3, 1, False)[0]
     for tol, want in [(1e-3, 4), (2e-4, 5), (1e-4, 6), (1e-6, 7), (1e-7, 8)]:
-        ns, beta = ctypes.c_int(), ctypes.c_double()
-        assert lib.finufft_amd_kernel_shape(ctypes.c_double(tol), 0, ctypes.c_double(2.0), 1,
-                                            ctypes.byref(ns), ctypes.byref(beta)) == 0
-        assert ns.value == want, tol
-        assert ns.value == kernel.kernel_shape(tol, 2.0, is_double=False)[0]
+        ier, ns, _, _ = _lib_kernel_shape(tol, 0, 2.0, 2)
+        assert ier == 0 and ns == want, tol
+        assert ns == kernel.kernel_shape(tol, 2.0, is_double=False)[0]
 
 
 @pytest.mark.parametrize("dim", [1, 2, 3])
