@@ -56,6 +56,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def finufft_probe():
+    """Whether a preinstalled reference CPU finufft is importable or on the
+    loader path (probed only; nothing is installed)."""
+    import importlib.util
+    import subprocess
+
+    found = []
+    if importlib.util.find_spec("finufft") is not None:
+        found.append("python module finufft")
+    try:
+        res = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10)
+        if "finufft" in res.stdout:
+            found.append("libfinufft on the loader path")
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return ", ".join(found) if found else "absent (no finufft module, no libfinufft)"
+
+
 def cpu_baseline(N, tol, M_full, M_sample, threads):
     """Times the oracle's CPU restatement on a sample of the workload and
     scales the spread/interp part to the full point count."""
@@ -88,6 +117,8 @@ def cpu_baseline(N, tol, M_full, M_sample, threads):
         "value": value,
         "unit": "NU pts/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
+        "reference_finufft": finufft_probe(),
         "kind": "port",
         "sample": (f"type1+type2 at N={N}^3, tol {tol}, on M={M_sample:.0e} of the {M_full:.0e} "
                    f"points; spread/interp time scaled linearly to M={M_full:.0e}, grid FFT "
@@ -179,28 +210,31 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
     log(f"batched workload: fused {t_batch * 1e3:.1f} ms ({info})")
     del b
 
-    # Learned options: the cost model picks the upsampling factor of every
-    # problem (among the options the fused executor runs); problems of equal
-    # choice form one batch each.
+    # Learned options: the cost model picks, per problem, among the options
+    # the fused executor runs (defaults, upsampling 1.25, the wider kernel);
+    # problems of equal choice form one batch each.
     learned = None
     from finufft_amd import costmodel
 
     model = costmodel.default_model()
     if model is not None:
-        choices = model.choose_many([(2, (int(n), int(n)), int(m), ntr, tol, "complex64")
-                                     for n, m in zip(Ns, Ms)], methods=(2,))
+        fused_arms = [i for i, a in enumerate(model.arms)
+                      if set(a) <= {"upsampfac", "eps_scale"}]
+        probs = [(2, (int(n), int(n)), int(m), ntr, tol, "complex64", costmodel.density_of(pts[i]))
+                 for i, (n, m) in enumerate(zip(Ns, Ms))]
+        choices = model.choose_many(probs, arms=fused_arms)
         del plans
         torch.cuda.empty_cache()
         lplans = []
         for i, (n, m) in enumerate(zip(Ns, Ms)):
-            opts = choices[i] or {}
-            p = Plan(2, (int(n), int(n)), n_trans=ntr, eps=tol, dtype="complex64", isign=-1,
+            eps, opts = model.plan_args(choices[i], 2, (int(n), int(n)), tol)
+            p = Plan(2, (int(n), int(n)), n_trans=ntr, eps=eps, dtype="complex64", isign=-1,
                      **opts)
             p.setpts(*pts[i])
             lplans.append(p)
         groups = {}
-        for i, o in enumerate(choices):
-            groups.setdefault(o.get("upsampfac", 0.0), []).append(i)
+        for i, a in enumerate(choices):
+            groups.setdefault(0 if a is None else a, []).append(i)
         batches = [(BatchPlan([lplans[i] for i in idx]), idx) for idx in groups.values()]
         for o in outs:
             o.zero_()
@@ -222,7 +256,7 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
             "ms": round(t_learned * 1e3, 3),
             "nu_evals_per_s": evals / t_learned,
             "error": e_learned,
-            "choices": {str(k): len(v) for k, v in groups.items()},
+            "choices": {json.dumps(model.arms[k]): len(v) for k, v in groups.items()},
         }
         log(f"batched workload: fused + learned {t_learned * 1e3:.1f} ms ({learned['choices']})")
         del batches, lplans
@@ -243,6 +277,87 @@ def batched_comparison(nprob, ntr, tol, reps, log_every=500):
         "graph": info["graph"],
         "learned": learned,
     }
+
+
+def spokes_3d(nspokes, per, seed, dev):
+    """Points on 3-D radial spokes: nspokes directions uniform on the sphere,
+    per samples each evenly spaced in radius from 0 to pi."""
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    u = torch.randn(nspokes, 3, generator=g, device=dev, dtype=torch.float64)
+    u = u / u.norm(dim=1, keepdim=True)
+    rad = torch.linspace(0, math.pi, per, device=dev, dtype=torch.float64)
+    p = (u[:, None, :] * rad[None, :, None]).reshape(-1, 3)
+    return [p[:, d].to(torch.float32).contiguous() for d in range(3)]
+
+
+def c3_comparison(pts, c, f, shape, tol, steps, err_samples):
+    """The headline problem with the library's default options ("plain")
+    and with the options the cost model selects from the problem and its
+    points' density ("selected"): ms per step (type 1 + type 2), NU pts/s and
+    the errors against the FP64 direct sum, for one point set."""
+    import torch
+
+    from finufft_amd import Plan
+    from finufft_amd import accuracy as acc
+    from finufft_amd import costmodel
+
+    dev = c.device
+    M = int(pts[0].numel())
+    model = costmodel.default_model()
+    dens = costmodel.density_of(pts)
+    out = {"density": [round(v, 3) for v in dens]}
+    variants = {"plain": (None, None)}
+    if model is not None:
+        a1 = model.choose(1, shape, M, 1, tol, "complex64", dens)
+        a2 = model.choose(2, shape, M, 1, tol, "complex64", dens)
+        variants["selected"] = (a1, a2)
+    r = torch.Generator(device="cpu")
+    r.manual_seed(5)
+    midx = torch.randint(0, math.prod(shape), (err_samples,), generator=r)
+    pidx = torch.randint(0, M, (err_samples,), generator=r)
+    ref1 = acc.type1_at(pts, c, 1, shape, midx)
+    ref2 = acc.type2_at(pts, f, -1, pidx)
+    fk_out = torch.empty(shape, dtype=torch.complex64, device=dev)
+    c_out = torch.empty(M, dtype=torch.complex64, device=dev)
+    for name, (a1, a2) in variants.items():
+        eps1, o1 = (tol, {}) if model is None else model.plan_args(a1, 1, shape, tol)
+        eps2, o2 = (tol, {}) if model is None else model.plan_args(a2, 2, shape, tol)
+        p1 = Plan(1, shape, eps=eps1, dtype="complex64", isign=1, **o1)
+        p2 = Plan(2, shape, eps=eps2, dtype="complex64", isign=-1, **o2)
+        p1.setpts(*pts)
+        p2.setpts(*pts)
+        p1.execute(c, out=fk_out)
+        p2.execute(f, out=c_out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            p1.execute(c, out=fk_out)
+            p2.execute(f, out=c_out)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        e1 = acc.relerr(fk_out.reshape(-1)[midx.to(dev)], ref1)
+        e2 = acc.relerr(c_out[pidx.to(dev)], ref2)
+        i1, i2 = p1.info(), p2.info()
+        out[name] = {
+            "ms_per_step": round(ms, 3), "nu_pts_per_s": 2 * M / (ms * 1e-3),
+            "error": {"type1": e1, "type2": e2},
+            "type1": {"options": model.arms[a1] if (model and a1 is not None) else {},
+                      "ns": i1["ns"], "gpu_method": i1["gpu_method"],
+                      "binsize": i1["binsize"], "upsampfac": i1.get("upsampfac")},
+            "type2": {"options": model.arms[a2] if (model and a2 is not None) else {},
+                      "ns": i2["ns"], "gpu_method": i2["gpu_method"],
+                      "binsize": i2["binsize"], "upsampfac": i2.get("upsampfac")},
+        }
+        log(f"c3 {name}: {ms:.2f} ms/step, errors {e1:.2e} / {e2:.2e}, "
+            f"type1 {out[name]['type1']}, type2 {out[name]['type2']}")
+        del p1, p2
+        torch.cuda.empty_cache()
+    if "selected" in out:
+        out["speedup"] = out["plain"]["ms_per_step"] / out["selected"]["ms_per_step"]
+    return out
 
 
 PROBE_MODES = ("ds_add_f32", "ds_add_f64", "ds_add_u32", "rmw_f32", "ds_read_b32",
@@ -301,6 +416,9 @@ def main(argv=None):
     ap.add_argument("--batched", type=int, default=2000,
                     help="problems of the uneven small-problem comparison (0: skip)")
     ap.add_argument("--batched-ntr", type=int, default=32)
+    ap.add_argument("--c3-compare", type=int, default=1,
+                    help="time the headline problem with the default and the cost-model-"
+                         "selected options on uniform and radial-spoke points (0: skip)")
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a tensor of this many MiB (a plain copy kernel of "
@@ -429,9 +547,27 @@ def main(argv=None):
         log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del src, dst
 
+    c3 = None
+    if args.c3_compare:
+        del p1, p2
+        torch.cuda.empty_cache()
+        steps = max(1, min(args.steps, 5))
+        c3 = {"uniform": c3_comparison(pts, c, f, shape, tol, steps, args.err_samples)}
+        del pts
+        torch.cuda.empty_cache()
+        nsp, per = (4096, 24414) if M == 100_000_000 else (4096, max(1, M // 4096))
+        spk = spokes_3d(nsp, per, 6, dev)
+        c3["radial"] = c3_comparison(spk, c[: spk[0].numel()], f, shape, tol, steps,
+                                     args.err_samples)
+        c3["radial"]["points"] = f"{nsp} spokes x {per} radii"
+        del spk
+        torch.cuda.empty_cache()
+    else:
+        del p1, p2, pts
+
     batched = None
     if args.batched > 0:
-        del p1, p2, pts, c, f, fk_out, c_out
+        del c, f, fk_out, c_out
         torch.cuda.empty_cache()
         batched = batched_comparison(args.batched, args.batched_ntr, 1e-5, args.batched_reps)
         torch.cuda.empty_cache()
@@ -472,6 +608,7 @@ def main(argv=None):
                       "type2": {k: round(v, 4) for k, v in s2.items()}},
         "setpts_ms": {"type1": round(t_set1 * 1e3, 2), "type2": round(t_set2 * 1e3, 2)},
         "error": {"type1": e1, "type2": e2, "samples": S},
+        "c3_plain_vs_selected": c3,
         "batched": batched,
     }
     print(json.dumps(line), flush=True)

@@ -18,6 +18,7 @@
 
 #include "batch.hpp"
 #include "kernel_math.hpp"
+#include "density.hpp"
 #include "mlp.hpp"
 #include "plan.hpp"
 
@@ -408,6 +409,22 @@ int finufft_amd_get_stage_times(void *plan, int is_double, double *ms) {
       as_plan<double>(plan)->timer().collect(ms);
     else
       as_plan<float>(plan)->timer().collect(ms);
+  });
+}
+
+int finufft_amd_density_stats(int dim, int64_t M, const void *x, const void *y, const void *z,
+                              int is_double, int g, double *stats, void *stream) {
+  return safe_call([&] {
+    if (!stats) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "density: null stats");
+    auto s = static_cast<hipStream_t>(stream);
+    if (is_double)
+      fa::density_stats<double>(dim, M, static_cast<const double *>(x),
+                                static_cast<const double *>(y), static_cast<const double *>(z),
+                                g, stats, s);
+    else
+      fa::density_stats<float>(dim, M, static_cast<const float *>(x),
+                               static_cast<const float *>(y), static_cast<const float *>(z), g,
+                               stats, s);
   });
 }
 

@@ -1,0 +1,17 @@
+// Point-density statistics for the cost model (see density.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fa {
+
+// stats[0] = max / mean, stats[1] = 99th percentile / mean, stats[2] =
+// occupied fraction of the counts over the g^dim cells of the folded points
+// (g^dim <= 2^20); synchronizes the stream.
+template <typename T>
+void density_stats(int dim, int64_t M, const T *x, const T *y, const T *z, int g, double *stats,
+                   hipStream_t stream);
+
+}  // namespace fa

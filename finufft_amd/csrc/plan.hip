@@ -520,6 +520,12 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
       permute<T>(idx_.get(), pts[d], sxyz_[d].get(), M_, stream_);
     }
   }
+  // Spreading with the LDS kernels reads the strengths in the sorted order.
+  sorted_c_ = type_ != 2 && opts_.gpu_method != 1 && !spread_gm_;
+  if (sorted_c_) {
+    inv_.reserve_at_least(std::max(M_, 1));
+    inverse_perm(idx_.get(), inv_.get(), M_, stream_);
+  }
   if (opts_.gpu_method != 1) {
     substart_.reserve_at_least(size_t(nb) + 1);
     // Column spreading runs whole bins in parity colors and only the bins
@@ -549,6 +555,15 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
   device_guard dg(opts_.gpu_device_id);
   ++setpts_gen_;
   if (M < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "negative number of points");
+  if (type_ == 3) {
+    // the reference's order: the targets' count, then the target arrays
+    // (setpts.cu:95-119); then (beyond the reference) the source arrays
+    if (N < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: negative number of targets");
+    if (int64_t(N) > MAX_NF) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: too many targets");
+    if (dim_ > 0 && s == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: s is null");
+    if (dim_ > 1 && t == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: t is null");
+    if (dim_ > 2 && u == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: u is null");
+  }
   if (M > 0 && (x == nullptr || (dim_ > 1 && y == nullptr) || (dim_ > 2 && z == nullptr)))
     fail(FINUFFT_ERR_INVALID_ARGUMENT, "null coordinate array");
   if (type_ != 3) {
@@ -567,11 +582,6 @@ void Plan<T>::setpts(int M, const T *x, const T *y, const T *z, int N, const T *
   t3_set_ = false;
   M_ = M;
   Nk_ = N;
-  if (N < 0) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: negative number of targets");
-  if (int64_t(N) > MAX_NF) fail(FINUFFT_ERR_NUM_NU_PTS_INVALID, "type 3: too many targets");
-  if (dim_ > 0 && s == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: s is null");
-  if (dim_ > 1 && t == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: t is null");
-  if (dim_ > 2 && u == nullptr) fail(FINUFFT_ERR_INVALID_ARGUMENT, "type 3: u is null");
   if (M == 0 || N == 0) {
     // No sources (the output is zero) or no targets (there is no output):
     // execute clears the output or returns, with no other launch.
@@ -688,6 +698,7 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.colstart = colstart_.get();
   c.dense_min = dense_min_;
   c.n_dense_sub = n_dense_sub_;
+  c.c_sorted = sorted_c_ ? 1 : 0;
   return c;
 }
 
@@ -708,6 +719,11 @@ void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
     zero<T>(fw, nftot_ * blk, stream_);
     timer_.end(stream_);
     timer_.begin(ST_SPREAD, stream_);
+    if (sorted_c_) {
+      csort_.reserve_at_least(size_t(M_) * blk);
+      scatter_sorted<T>(c, nullptr, inv_.get(), csort_.get(), M_, blk, stream_);
+      c = csort_.get();
+    }
     spread<T>(ctx, c, fw, blk, stream_);
     timer_.end(stream_);
     if (spread_only_) continue;
@@ -757,7 +773,10 @@ void Plan<T>::exec_type3(C *d_c, C *d_fk) {
     const int blk = std::min(ntransf_ - i * batchsize_, batchsize_);
     const C *c = d_c + int64_t(i) * batchsize_ * M_;
     C *fk = d_fk + int64_t(i) * batchsize_ * Nk_;
-    cmul<T>(c, prephase_.get(), cpre_.get(), M_, blk, stream_);
+    if (sorted_c_)  // the prephase applied while scattering into the sorted order
+      scatter_sorted<T>(c, prephase_.get(), inv_.get(), cpre_.get(), M_, blk, stream_);
+    else
+      cmul<T>(c, prephase_.get(), cpre_.get(), M_, blk, stream_);
     zero<T>(fw_.get(), nftot_ * blk, stream_);
     spread<T>(ctx, cpre_.get(), fw_.get(), blk, stream_);
     inner_->exec_type2(fk, fw_.get(), inner_->iflag_, blk);

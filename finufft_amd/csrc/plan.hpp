@@ -122,6 +122,12 @@ class Plan {
   // sort / subproblems
   BinGeom geom_;
   dbuf<int> idx_, binstart_, substart_, sub2bin_, colstart_;
+  // Spreading plans of the LDS methods: the inverse sort permutation, and
+  // the strengths of an execute scattered into the sorted order through it
+  // (the kernels then read them coalesced instead of gathering c[idx[q]]).
+  dbuf<int> inv_;
+  dbuf<typename cplx_of<T>::type> csort_;
+  bool sorted_c_ = false;
   dbuf<T> sxyz_[3];  // coordinates in sorted order
   bool colored_ = false;
   int sub_capacity_ = 0;

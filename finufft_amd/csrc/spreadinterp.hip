@@ -382,13 +382,13 @@ __device__ __forceinline__ void spread_into_tile(const SpreadCtx<T> &ctx,
   for (int i = threadIdx.x; i < st.npts; i += BIN_THREADS) {
     const int q = st.first + i;
     const int j = ctx.idx[q];
+    const cplx_t<T> cv = c[ctx.c_sorted ? q : j];
     T ker[DIM][NS];
     int l[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       l[d] = st.local_start(start_and_kernel<T, NS>(coord(ctx, d, q, j), ctx.nf[d], ker[d], ctx.kp),
                             d, ctx.nf[d]);
-    const cplx_t<T> cv = c[j];
     if constexpr (DIM == 1) {
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
@@ -615,8 +615,9 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
   // (clearing the tile), and each next batch's strengths while the current
   // batch is walked, so that memory latency overlaps work.
   fetch(u0 + lane);
+  const bool csort = ctx.c_sorted != 0;
   float2 fc = make_float2(0.f, 0.f);
-  if (u0 + lane < u1) fc = c[fj];
+  if (u0 + lane < u1) fc = c[csort ? u0 + lane : fj];
   start();
 
   float ar[ZD], ai[ZD];
@@ -631,6 +632,9 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
     const float x = fx, y = fy, z = fz;
     const float2 cv = fc;
     fetch(p + 64 + lane);
+    // the next batch's strengths: at once when they are in the sorted order,
+    // else after the staging below (they need the next sorted index)
+    if (csort && p + 64 + lane < u1) fc = c[p + 64 + lane];
     int posv = 0;  // lz + 64 * sub-column of this lane's point
     if (lane < n) {
       float kx[NS], ky[NS], kz[NS];
@@ -674,7 +678,7 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
       r[6] = make_float4(wy8[0], wy8[1], wy8[2], wy8[3]);
       r[7] = make_float4(wy8[4], wy8[5], wy8[6], wy8[7]);
     }
-    if (p + 64 + lane < u1) fc = c[fj];  // the next batch's strengths
+    if (!csort && p + 64 + lane < u1) fc = c[fj];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");

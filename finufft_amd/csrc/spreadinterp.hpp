@@ -71,6 +71,9 @@ struct SpreadCtx {
 
   int colS = 0;
   const int *colstart = nullptr;
+  // Spreading strengths already in the sorted order (c[q] for the q-th
+  // sorted point rather than c[idx[q]]): LDS spreading kernels only.
+  int c_sorted = 0;
   int dense_min = 0;
   int n_dense_sub = 0;
 };

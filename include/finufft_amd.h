@@ -73,9 +73,9 @@ FINUFFT_EXPORT int finufft_amd_batch_set_graph(finufft_amd_batch batch, int use_
  *            mean_b 0.5 sum_j mask[b][j] (y[b][j] - target[b][j])^2 (grad and
  *            loss must be zeroed by the caller);
  *   adam:    one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at step >= 1. */
-#define FINUFFT_AMD_MLP_IN 8
+#define FINUFFT_AMD_MLP_IN 11
 #define FINUFFT_AMD_MLP_HIDDEN 32
-#define FINUFFT_AMD_MLP_OUT 4
+#define FINUFFT_AMD_MLP_OUT 8
 #define FINUFFT_AMD_MLP_NPARAM                                                       \
   (FINUFFT_AMD_MLP_HIDDEN * FINUFFT_AMD_MLP_IN + FINUFFT_AMD_MLP_HIDDEN +            \
    FINUFFT_AMD_MLP_HIDDEN * FINUFFT_AMD_MLP_HIDDEN + FINUFFT_AMD_MLP_HIDDEN +        \
@@ -87,6 +87,18 @@ FINUFFT_EXPORT int finufft_amd_mlp_grad(const float *params, int n, const float 
                                         float *loss, void *stream);
 FINUFFT_EXPORT int finufft_amd_mlp_adam(float *params, const float *grad, float *m1, float *m2,
                                         int step, float lr, void *stream);
+
+/* Point-density statistics for the cost model; no reference counterpart
+ * (the reference's method tables do not look at the points,
+ * src/cuda/heuristics.cu:270-281). x, y, z: device coordinate arrays of M
+ * points (fp32, or fp64 with is_double), as for setpts. Each coordinate is
+ * folded into [0, g) and the points counted per cell of the g^dim grid
+ * (g^dim <= 2^20); stats[3] receives max / mean, 99th percentile / mean and
+ * the occupied fraction of the cell counts. Synchronizes `stream` (a
+ * hipStream_t, NULL: the null stream). */
+FINUFFT_EXPORT int finufft_amd_density_stats(int dim, int64_t M, const void *x, const void *y,
+                                             const void *z, int is_double, int g,
+                                             double *stats, void *stream);
 
 /* Plan introspection; no reference counterpart (the reference exposes these
  * values only through opts.debug printing, src/cuda/makeplan.cu:116-118).

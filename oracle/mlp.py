@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, H, OUT = 8, 32, 4
+IN, H, OUT = 11, 32, 8
 SHAPES = [("W1", (H, IN)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (OUT, H)),
           ("b3", (OUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)

@@ -159,12 +159,23 @@ int main(int argc, char **argv) {
     for (int64_t i = 0; i < int64_t(M) * blk; ++i) c[i] = {T(NAN), T(NAN)};
   }
 
+  // EMU_CSORTED=1: the LDS spreading kernels read the strengths in the
+  // sorted order (ctx.c_sorted), as the library's spreading plans do
+  C *cs = c;
+  if (spreads && var != "gm" && std::getenv("EMU_CSORTED") &&
+      std::string(std::getenv("EMU_CSORTED")) == "1") {
+    cs = alloc<C>(size_t(M) * blk);
+    for (int t = 0; t < blk; ++t)
+      for (int i = 0; i < M; ++i) cs[int64_t(t) * M + i] = c[int64_t(t) * M + idx[i]];
+    ctx.c_sorted = 1;
+  }
+
   if (var == "gm") {
     const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
     emu_launch(g, SPREAD_THREADS, 0, [&] { spread_gm_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
   } else if (var == "sub") {
     emu_launch(cap, BIN_THREADS, ctx.lds_spread,
-               [&] { spread_sub_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
+               [&] { spread_sub_kernel<T, DIM, NS>(ctx, cs, fw, nftot); }, blk);
   } else if (var == "colored") {
     ctx.colored = 1;
     for (int color = 0; color < (1 << DIM); ++color) {
@@ -175,7 +186,7 @@ int main(int argc, char **argv) {
         n *= (nb[d] - col[d] + 1) / 2;
       }
       emu_launch(n, BIN_THREADS, ctx.lds_spread, [&] {
-        spread_bin_colored_kernel<T, DIM, NS>(ctx, c, fw, nftot, col[0], col[1], col[2]);
+        spread_bin_colored_kernel<T, DIM, NS>(ctx, cs, fw, nftot, col[0], col[1], col[2]);
       }, blk);
     }
   } else if (col) {
@@ -198,7 +209,7 @@ int main(int argc, char **argv) {
                      [&] { spread_col_sub_kernel<NS>(cx, cc, ff, nftot); }, blk);
       }
     };
-    launch_col(ctx, c, fw);
+    launch_col(ctx, cs, fw);
   } else if (var == "bin" || var == "binsub") {
     emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
                [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
@@ -264,6 +275,7 @@ int main(int argc, char **argv) {
   std::free(substart);
   std::free(sub2bin);
   std::free(colstart);
+  if (cs != c) std::free(cs);
   std::free(c);
   std::free(fw);
   return 0;

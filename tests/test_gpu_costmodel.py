@@ -54,7 +54,8 @@ def test_training_fits(torch_gpu):
     r = np.random.default_rng(2)
     x = r.normal(0, 1, (400, omlp.IN)).astype(np.float32)
     t = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1], x[:, 2] ** 2 * 0.3, x[:, 3] - x[:, 4],
-                  np.cos(x[:, 5])], axis=1).astype(np.float32)
+                  np.cos(x[:, 5]), 0.5 * x[:, 6] * x[:, 7], np.tanh(x[:, 8]),
+                  x[:, 9] + 0.2 * x[:, 10], np.abs(x[:, 0]) * 0.5], axis=1).astype(np.float32)
     m = np.ones_like(t)
     l0 = omlp.loss(costmodel.init_params(0).astype(np.float64), x.astype(np.float64),
                    t.astype(np.float64), m)
@@ -65,53 +66,112 @@ def test_training_fits(torch_gpu):
     assert abs(loss - l_ref) < 0.05 * l0  # the last reported loss is that of the last step
 
 
+def test_density_stats_vs_numpy(torch_gpu):
+    """The density kernel's counts reproduce a numpy histogram of the folded
+    points (uniform, clustered and spoke sets, 2-D and 3-D, both precisions)."""
+    from finufft_amd import costmodel
+
+    torch = torch_gpu
+    for dim in (2, 3):
+        for kind in ("uniform", "clustered", "spokes"):
+            for real in ("float32", "float64"):
+                pts = costmodel.make_points(kind, dim, 50_000, seed=7, real=real)
+                got = costmodel.density_of(pts)
+                g = costmodel.DENSITY_GRID[dim]
+                h = [p.double().cpu().numpy() for p in pts]
+                cells = [np.clip(np.floor(((a + np.pi) % (2 * np.pi)) / (2 * np.pi) * g), 0,
+                                 g - 1) for a in h]
+                counts = np.bincount(
+                    np.ravel_multi_index([c.astype(int) for c in cells], (g,) * dim),
+                    minlength=g ** dim)
+                mean = len(h[0]) / g ** dim
+                want = (counts.max() / mean, np.sort(counts)[int(0.99 * g ** dim)] / mean,
+                        np.count_nonzero(counts) / g ** dim)
+                print(dim, kind, real, got, want)
+                # the fold's rounding can move a point at a cell edge: loose
+                assert got[0] == pytest.approx(want[0], rel=0.05, abs=2 / mean)
+                assert got[1] == pytest.approx(want[1], rel=0.05, abs=2 / mean)
+                assert got[2] == pytest.approx(want[2], rel=0.02)
+                if kind == "spokes":
+                    assert got[0] > 20  # far from uniform
+    assert costmodel.density_of([torch.empty(0, device="cuda")] * 2) == (0.0, 0.0, 0.0)
+
+
 def test_choose_and_fallback(torch_gpu):
     from finufft_amd import costmodel
 
     p = costmodel.init_params(3)
     lo = np.zeros(costmodel.NIN, np.float32)
     hi = np.ones(costmodel.NIN, np.float32)
-    model = costmodel.CostModel(p, np.zeros(costmodel.NIN), np.ones(costmodel.NIN), lo, hi)
+    key = costmodel.problem_class(2, 2, 1e-5, "complex64")
+    model = costmodel.CostModel(p, np.zeros(costmodel.NIN), np.ones(costmodel.NIN), lo, hi,
+                                {key: []})
+    dens = (1.5, 1.3, 1.0)
     # far outside the (tiny) training range: the library's heuristic
-    assert model.choose(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64") == {}
-    f = costmodel.features(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64")
+    assert model.choose(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", dens) is None
+    f = costmodel.features(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", dens)
     model.lo, model.hi = f - 1, f + 1
-    choice = model.choose(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64")
-    assert choice in costmodel.ARMS
+    choice = model.choose(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", dens)
+    assert 0 <= choice < costmodel.NOUT
     # a rejected option is never chosen
-    best = costmodel.ARMS.index(choice)
-    model.rejected = {costmodel.problem_class(2, 2, 1e-5, "complex64"): [best]}
-    assert model.choose(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64") != choice
+    model.rejected = {key: [choice]}
+    assert model.choose(2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", dens) != choice
+
+
+@pytest.mark.parametrize("kind", ["uniform", "spokes"])
+def test_selected_options_meet_tol_c3_shape(torch_gpu, kind):
+    """The headline problem's shape at a size the FP64 direct sum checks
+    fully: 3-D types 1 and 2, fp32, tol 1e-6. The committed model's choice
+    (or the heuristic, where it declines) keeps the error within tol."""
+    from finufft_amd import costmodel
+
+    torch = torch_gpu
+    model = costmodel.default_model()
+    if model is None:
+        pytest.skip("no committed cost-model weights")
+    pts = costmodel.make_points(kind, 3, 400_000, seed=11)
+    dens = costmodel.density_of(pts)
+    for ttype in (1, 2):
+        a = model.choose(ttype, (48, 48, 48), 400_000, 1, 1e-6, "complex64", dens)
+        arm = model.arms[a] if a is not None else {}
+        ms, err = costmodel.time_arm(ttype, (48, 48, 48), pts, 1, 1e-6, "complex64", arm,
+                                     reps=2, err_samples=256)
+        print(kind, ttype, dens, arm, ms, err)
+        assert err <= 1e-6
 
 
 def test_committed_model_gain(torch_gpu):
     """On problems the model was not trained on, the model's options are not
-    slower in total than the default ones and keep the error."""
+    slower in total than the default ones, keep the error within tol, and
+    gain on the many-vector problems."""
     from finufft_amd import costmodel
 
     model = costmodel.default_model()
     if model is None:
         pytest.skip("no committed cost-model weights")
-    cfgs = [(2, (256, 256), 50_000, 32, 1e-5, "complex64"),
-            (2, (480, 480), 3_000, 32, 1e-5, "complex64"),
-            (2, (48, 48), 40_000, 32, 1e-5, "complex64"),
-            (1, (64, 64, 64), 1_000_000, 1, 1e-6, "complex64"),
-            (1, (40, 40, 40), 20_000, 32, 1e-3, "complex64"),
-            (2, (100, 100), 8_000, 32, 1e-6, "complex64"),
-            (1, (200, 200), 150_000, 4, 1e-9, "complex128")]
+    cfgs = [(2, (256, 256), 50_000, 32, 1e-5, "complex64", "uniform"),
+            (2, (480, 480), 3_000, 32, 1e-5, "complex64", "uniform"),
+            (2, (48, 48), 40_000, 32, 1e-5, "complex64", "clustered"),
+            (1, (64, 64, 64), 1_000_000, 1, 1e-6, "complex64", "uniform"),
+            (1, (40, 40, 40), 20_000, 32, 1e-3, "complex64", "spokes"),
+            (2, (100, 100), 8_000, 32, 1e-6, "complex64", "uniform"),
+            (1, (200, 200), 150_000, 4, 1e-9, "complex128", "clustered")]
     t_def = t_mod = 0.0
     log_gain = 0.0
-    for i, c in enumerate(cfgs):
-        opts = model.choose(*c) or costmodel.ARMS[0]
-        ms0, e0 = costmodel.time_arm(*c, costmodel.ARMS[0], seed=100 + i)
-        ms1, e1 = costmodel.time_arm(*c, opts, seed=100 + i)
-        print(c, opts, ms0, ms1, e0, e1)
-        assert e1 <= max(c[4], 1.5 * e0) * 1.5
+    for i, (ttype, nm, M, ntr, tol, dtype, kind) in enumerate(cfgs):
+        real = "float64" if dtype == "complex128" else "float32"
+        pts = costmodel.make_points(kind, len(nm), M, seed=200 + i, real=real)
+        dens = costmodel.density_of(pts)
+        a = model.choose(ttype, nm, M, ntr, tol, dtype, dens)
+        arm = model.arms[a] if a is not None else {}
+        ms0, e0 = costmodel.time_arm(ttype, nm, pts, ntr, tol, dtype, {}, seed=100 + i)
+        ms1, e1 = costmodel.time_arm(ttype, nm, pts, ntr, tol, dtype, arm, seed=100 + i)
+        print(ttype, nm, M, ntr, tol, kind, arm, ms0, ms1, e0, e1)
+        assert e1 <= tol or (a is None and e1 <= 2 * tol)
         t_def += ms0
         t_mod += ms1
         log_gain += math.log(ms0 / ms1)
     gmean = math.exp(log_gain / len(cfgs))
     print("default", t_def, "model", t_mod, "total gain", t_def / t_mod, "per-problem gain", gmean)
     assert t_mod <= 1.1 * t_def
-    # the gain: many-vector problems of these sizes run faster in one pass
     assert gmean > 1.05
