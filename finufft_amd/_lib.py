@@ -102,6 +102,9 @@ def _declare(lib):
     lib.finufft_amd_density_stats.argtypes = [c_int, c_int64, vp, vp, vp, c_int, c_int,
                                               POINTER(c_double), vp]
     lib.finufft_amd_density_stats.restype = c_int
+    lib.finufft_amd_density_stats_noalloc.argtypes = [c_int, c_int64, vp, vp, vp, c_int, c_int,
+                                                      vp, vp]
+    lib.finufft_amd_density_stats_noalloc.restype = c_int
     lib.finufft_amd_lds_probe.argtypes = [c_int, c_int, c_int, POINTER(c_double),
                                           POINTER(c_double)]
     lib.finufft_amd_lds_probe.restype = c_int

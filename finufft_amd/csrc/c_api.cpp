@@ -428,6 +428,22 @@ int finufft_amd_density_stats(int dim, int64_t M, const void *x, const void *y, 
   });
 }
 
+int finufft_amd_density_stats_noalloc(int dim, int64_t M, const void *x, const void *y,
+                                      const void *z, int is_double, int g, double *d_stats,
+                                      void *stream) {
+  return safe_call([&] {
+    auto s = static_cast<hipStream_t>(stream);
+    if (is_double)
+      fa::density_stats_noalloc<double>(dim, M, static_cast<const double *>(x),
+                                        static_cast<const double *>(y),
+                                        static_cast<const double *>(z), g, d_stats, s);
+    else
+      fa::density_stats_noalloc<float>(dim, M, static_cast<const float *>(x),
+                                       static_cast<const float *>(y),
+                                       static_cast<const float *>(z), g, d_stats, s);
+  });
+}
+
 int finufft_amd_lds_probe(int mode, int blocks, int reps, double *ms, double *wave_ops) {
   return safe_call([&] {
     if (!ms || !wave_ops) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "null output");

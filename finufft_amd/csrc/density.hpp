@@ -14,4 +14,12 @@ template <typename T>
 void density_stats(int dim, int64_t M, const T *x, const T *y, const T *z, int g, double *stats,
                    hipStream_t stream);
 
+// The same statistics with no allocation and no synchronization: counts in
+// a file-scope device array of density_noalloc_max_cells() cells, the result
+// written to the device array d_stats[3] on `stream`.
+template <typename T>
+void density_stats_noalloc(int dim, int64_t M, const T *x, const T *y, const T *z, int g,
+                           double *d_stats, hipStream_t stream);
+int64_t density_noalloc_max_cells();
+
 }  // namespace fa

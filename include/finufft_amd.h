@@ -100,6 +100,18 @@ FINUFFT_EXPORT int finufft_amd_density_stats(int dim, int64_t M, const void *x, 
                                              const void *z, int is_double, int g,
                                              double *stats, void *stream);
 
+/* The statistics of finufft_amd_density_stats without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). The counts
+ * live in a library-owned device array of 2^18 cells (1 MiB per GPU, held
+ * until the process ends), so g^dim <= 2^18 (else 21, invalid argument); the
+ * three statistics are written to the DEVICE array d_stats[3] on `stream`,
+ * with no host synchronization. Calls that overlap on one GPU share the
+ * array and race: serialize them (one stream). */
+FINUFFT_EXPORT int finufft_amd_density_stats_noalloc(int dim, int64_t M, const void *x,
+                                                     const void *y, const void *z, int is_double,
+                                                     int g, double *d_stats, void *stream);
+
 /* Plan introspection; no reference counterpart (the reference exposes these
  * values only through opts.debug printing, src/cuda/makeplan.cu:116-118).
  * iout[16]: type, dim, ns, nf1, nf2, nf3, gpu_method, binsize x/y/z,
