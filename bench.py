@@ -24,7 +24,13 @@ Extra fields:
                 algorithmic bytes per launch over its average launch time
                 (the stage time, timed with HIP events on the plan's stream
                 over the timed region, over its launches: one per parity
-                color for colored spreading);
+                color for colored spreading); traffic, the HBM bytes per
+                launch the kernel moves, and its limiter come from the
+                committed counter summary of this workload (used when the
+                summary's kernel time matches this run's within 25%);
+  other_configs the 2-D type 1 + 2 workloads (uniform and clustered points)
+                and the 3-D differentiation workload, each with its time,
+                throughput, errors and spread / interp roofline figures;
   stages        average ms per execute of every stage of both plans;
   error         relative l2 error against an FP64 direct sum on the GPU at
                 sampled modes (type 1) and sampled points (type 2);
@@ -50,6 +56,12 @@ if str(ROOT) not in sys.path:
 
 METRIC = "NU pts/s 3D type1/2 fp32 @tol1e-6; batched+learned vs plain speedup, same err"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the headline kernels, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs of this workload (calibrated against a plain copy), with
+# the command that made them and what the SQ counters say bounds each kernel
+TRAFFIC = ROOT / "profiles" / "hbm_traffic_3d_fp32_M1e8_N256.json"
+STAGE_KERNEL = {"spread (type 1)": "spread_col_colored_kernel<7>",
+                "interp (type 2)": "interp_bin_kernel<float, 3, 7>"}
 
 
 def log(msg):
@@ -83,6 +95,20 @@ def finufft_probe():
     except (OSError, subprocess.SubprocessError):
         pass
     return ", ".join(found) if found else "absent (no finufft module, no libfinufft)"
+
+
+def measured_traffic(stage, ms_per_launch):
+    """(HBM bytes per launch, source, limiter) of the stage's kernel from the
+    committed counter summary, or (None, None, None) when it has no entry.
+    The summary's kernel time is given too: a reading taken at a kernel time
+    far from this run's is stale and not used."""
+    try:
+        rec = json.loads(TRAFFIC.read_text())["kernels"][STAGE_KERNEL[stage]]
+    except (OSError, KeyError, ValueError):
+        return None, None, None
+    if not 0.8 <= rec["ms_per_launch"] / ms_per_launch <= 1.25:
+        return None, None, None
+    return int(rec["hbm_bytes_per_launch"]), TRAFFIC.name, rec.get("limiter")
 
 
 def cpu_baseline(N, tol, M_full, M_sample, threads):
@@ -360,9 +386,171 @@ def c3_comparison(pts, c, f, shape, tol, steps, err_samples):
     return out
 
 
+def gaussian_mixture_2d(M, seed, dev):
+    """M points from a mixture of 16 Gaussians (sigma 0.05 pi, means uniform
+    on [-pi, pi)^2, equal weights); the library folds points outside the
+    period."""
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    means = torch.rand(16, 2, generator=g, device=dev) * (2 * math.pi) - math.pi
+    comp = torch.randint(0, 16, (M,), generator=g, device=dev)
+    p = means[comp] + 0.05 * math.pi * torch.randn(M, 2, generator=g, device=dev)
+    return [p[:, d].contiguous() for d in range(2)]
+
+
+def _stage_roofline(M, dim, nf, ms, which):
+    """Algorithmic bytes of a spread or interp stage (one vector) and the
+    fraction of the HBM peak at the stage's measured time."""
+    nbytes = (M * (4 + 4 * dim + 8) + 8 * nf) if which == "spread" else (
+        8 * nf + M * (4 + 4 * dim) + 8 * M)
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"stage": which, "ms": round(ms, 4), "alg_bytes": nbytes, "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def type12_config(pts, shape, tol, steps, err_samples, seed):
+    """Type 1 + type 2 (fp32, default options) on the given points: ms per
+    step, NU pts/s, errors at sampled modes / points, and the spread and
+    interp stages against the HBM roofline."""
+    import torch
+
+    from finufft_amd import Plan
+    from finufft_amd import accuracy as acc
+
+    dev = pts[0].device
+    M, dim = int(pts[0].numel()), len(shape)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    c = torch.complex(torch.randn(M, generator=g, device=dev),
+                      torch.randn(M, generator=g, device=dev))
+    f = torch.complex(torch.randn(shape, generator=g, device=dev),
+                      torch.randn(shape, generator=g, device=dev))
+    p1 = Plan(1, shape, eps=tol, dtype="complex64", isign=1)
+    p2 = Plan(2, shape, eps=tol, dtype="complex64", isign=-1)
+    p1.setpts(*pts)
+    p2.setpts(*pts)
+    fk = torch.empty(shape, dtype=torch.complex64, device=dev)
+    co = torch.empty(M, dtype=torch.complex64, device=dev)
+    p1.execute(c, out=fk)
+    p2.execute(f, out=co)
+    p1.set_stage_timing(True)
+    p2.set_stage_timing(True)
+    p1.stage_times()
+    p2.stage_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p1.execute(c, out=fk)
+        p2.execute(f, out=co)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    s1 = {k: v / steps for k, v in p1.stage_times().items()}
+    s2 = {k: v / steps for k, v in p2.stage_times().items()}
+    r = torch.Generator(device="cpu")
+    r.manual_seed(seed + 1)
+    midx = torch.randint(0, math.prod(shape), (err_samples,), generator=r)
+    pidx = torch.randint(0, M, (err_samples,), generator=r)
+    e1 = acc.relerr(fk.reshape(-1)[midx.to(dev)], acc.type1_at(pts, c, 1, shape, midx))
+    e2 = acc.relerr(co[pidx.to(dev)], acc.type2_at(pts, f, -1, pidx))
+    i1, i2 = p1.info(), p2.info()
+    out = {
+        "ms_per_step": round(ms, 3), "nu_pts_per_s": 2 * M / (ms * 1e-3),
+        "error": {"type1": e1, "type2": e2},
+        "spread": _stage_roofline(M, dim, math.prod(i1["nf"]), s1["spread"], "spread"),
+        "interp": _stage_roofline(M, dim, math.prod(i2["nf"]), s2["interp"], "interp"),
+        "type1": {"ns": i1["ns"], "binsize": i1["binsize"], "nf": i1["nf"]},
+        "type2": {"ns": i2["ns"], "binsize": i2["binsize"], "nf": i2["nf"]},
+    }
+    del p1, p2
+    torch.cuda.empty_cache()
+    return out
+
+
+def differentiation_config(steps, err_samples):
+    """3-D type 2 with gradients (fp32, tol 1e-6, N = 128^3): 1000 spokes x
+    2000 radii on [0, pi] plus uniform jitter in [-1e-3, 1e-3], loss
+    0.5 ||T2(f, x) - y||^2, forward + backward (gradients in f and in the
+    three coordinate arrays) per step; the forward's error against the FP64
+    direct sum at sampled points. (Finite-difference and FP64 gradient checks
+    are in the GPU tests.)"""
+    import torch
+
+    from finufft_amd import accuracy as acc
+    from finufft_amd.autograd import NufftType2
+
+    dev = torch.device("cuda", 0)
+    N, tol = 128, 1e-6
+    sp = spokes_3d(1000, 2000, 7, dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(8)
+    pts = [(p + (torch.rand(p.numel(), generator=g, device=dev) * 2e-3 - 1e-3)).contiguous()
+           for p in sp]
+    M = int(pts[0].numel())
+    f = torch.complex(torch.randn((N, N, N), generator=g, device=dev),
+                      torch.randn((N, N, N), generator=g, device=dev))
+    y = torch.complex(torch.randn(M, generator=g, device=dev),
+                      torch.randn(M, generator=g, device=dev))
+    mod = NufftType2((N, N, N), eps=tol)
+
+    def step():
+        fr = f.clone().requires_grad_(True)
+        xr = [p.clone().requires_grad_(True) for p in pts]
+        cv = mod(fr, *xr)
+        loss = 0.5 * (cv - y).abs().pow(2).sum()
+        loss.backward()
+        return cv.detach()
+
+    cv = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    r = torch.Generator(device="cpu")
+    r.manual_seed(9)
+    pidx = torch.randint(0, M, (err_samples,), generator=r)
+    err = acc.relerr(cv[pidx.to(dev)], acc.type2_at(pts, f, -1, pidx))
+    return {"workload": "3D type-2 fp32 tol 1e-6, N=128^3, M=2e6 (1000 spokes x 2000 radii, "
+                        "jitter 1e-3), forward + backward (gradients in f and x)",
+            "ms_per_step": round(ms, 3), "nu_pts_per_s": M / (ms * 1e-3), "error_forward": err}
+
+
+def other_configs(steps, err_samples):
+    """The other single-GPU workloads of the metric's configuration list:
+    2-D type 1 + 2 at M = 1e7, N = 1000^2, fp32 tol 1e-5 on uniform and
+    Gaussian-mixture points, and the 3-D differentiation problem."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    out = {}
+    g = torch.Generator(device=dev)
+    g.manual_seed(10)
+    M = 10_000_000
+    uni = [torch.rand(M, generator=g, device=dev) * (2 * math.pi) - math.pi for _ in range(2)]
+    out["2d_uniform"] = type12_config(uni, (1000, 1000), 1e-5, steps, err_samples, 11)
+    out["2d_uniform"]["workload"] = "2D type1+type2 fp32 tol 1e-5, M=1e7 uniform, N=1000^2"
+    del uni
+    mix = gaussian_mixture_2d(M, 12, dev)
+    out["2d_clustered"] = type12_config(mix, (1000, 1000), 1e-5, steps, err_samples, 13)
+    out["2d_clustered"]["workload"] = ("2D type1+type2 fp32 tol 1e-5, M=1e7 from 16 Gaussians "
+                                       "(sigma 0.05 pi), N=1000^2")
+    del mix
+    torch.cuda.empty_cache()
+    out["3d_differentiation"] = differentiation_config(steps, err_samples)
+    for k, v in out.items():
+        log(f"{k}: {v['ms_per_step']} ms/step, {v['nu_pts_per_s']:.3e} NU pts/s")
+    torch.cuda.empty_cache()
+    return out
+
+
 PROBE_MODES = ("ds_add_f32", "ds_add_f64", "ds_add_u32", "rmw_f32", "ds_read_b32",
                "ds_read_b32_uniform", "ds_read_b128", "ds_read_b128_uniform", "v_fma_f32",
-               "ds_add_f32_rotating", "ds_add_f64_rotating")
+               "ds_add_f32_rotating", "ds_add_f64_rotating", "ds_read_b64_stride1",
+               "ds_read_b64_stride2", "ds_read_b64_stride16", "ds_read_b64_stride32",
+               "ds_read_b64_stride64", "ds_add_f32_denorm_flush")
 
 
 def lds_probe():
@@ -420,6 +608,9 @@ def main(argv=None):
                     help="time the headline problem with the default and the cost-model-"
                          "selected options on uniform and radial-spoke points (0: skip)")
     ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--others", type=int, default=1,
+                    help="also time the 2-D (uniform, clustered) and 3-D differentiation "
+                         "workloads (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a tensor of this many MiB (a plain copy kernel of "
                          "known size, the reference point for HBM counter readings)")
@@ -520,10 +711,11 @@ def main(argv=None):
             ("interp (type 2)", s2["interp"], interp_bytes, 1)]
     name, ms, nbytes, nl = max(cand, key=lambda t: t[1])
     achieved = nbytes / (ms * 1e-3) / 1e9
+    traffic, source, limiter = measured_traffic(name, ms / nl)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": name, "launches": nl, "ms_per_launch": round(ms / nl, 4),
-                "alg_bytes": nbytes // nl}
+                "alg_bytes": nbytes // nl, "traffic_source": source, "limiter": limiter}
 
     # accuracy against the FP64 direct sum at sampled modes / points
     r = torch.Generator(device="cpu")
@@ -564,6 +756,10 @@ def main(argv=None):
         torch.cuda.empty_cache()
     else:
         del p1, p2, pts
+
+    others = None
+    if args.others:
+        others = other_configs(max(1, min(args.steps, 5)), min(args.err_samples, 64))
 
     batched = None
     if args.batched > 0:
@@ -609,6 +805,7 @@ def main(argv=None):
         "setpts_ms": {"type1": round(t_set1 * 1e3, 2), "type2": round(t_set2 * 1e3, 2)},
         "error": {"type1": e1, "type2": e2, "samples": S},
         "c3_plain_vs_selected": c3,
+        "other_configs": others,
         "batched": batched,
     }
     print(json.dumps(line), flush=True)

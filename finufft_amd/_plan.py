@@ -147,7 +147,7 @@ class Plan:
             "type": v[0], "dim": v[1], "ns": v[2], "nf": tuple(v[3:6]), "gpu_method": v[6],
             "binsize": tuple(v[7:10]), "batchsize": v[10], "M": v[11], "nbins": v[12],
             "col_side": v[13], "colored": v[14], "dense_subproblems": v[15],
-            "beta": dout[0], "upsampfac": dout[1],
+            "beta": dout[0], "upsampfac": dout[1], "np": int(dout[2]),
         }
 
     STAGES = ("zero", "spread", "fft", "deconvolve", "interp", "type3")

@@ -1,7 +1,8 @@
 """Differentiable NUFFTs: torch.autograd.Functions over the HIP library.
 
 Type 2, c_j = sum_k f_k exp(i s k . x_j):
-  grad_f = A^H g                  (the plan's adjoint, a type-1 transform with -s)
+  grad_f = A^H g                  (a type-1 transform with -s, on a type-1
+                                   plan of its own for NufftType2)
   grad_x = Re(conj(g) * dc/dx)    (dc/dx from the derivative-kernel interpolation)
 Type 1, f_k = sum_j c_j exp(i s k . x_j):
   grad_c = A^H g                  (the plan's adjoint, a type-2 transform with -s)
@@ -20,9 +21,13 @@ forward calls of one module before a backward).
 
 from __future__ import annotations
 
+import itertools
+
 import torch
 
 from ._plan import Plan
+
+_FORWARD_IDS = itertools.count()
 
 
 def _sum_batch(t, nbatch):
@@ -40,12 +45,39 @@ def _points_of(ctx):
     return plan, data
 
 
+class _AdjointSpreader:
+    """The adjoint of a type-2 plan applied by a type-1 plan of its own
+    (isign flipped; the same modes, vectors, tolerance and options): on the
+    same points its spreading runs the type-1 kernels and bins (the column
+    spreader for 3-D fp32), which stay fast on very dense point sets (the
+    centre of radial spokes) where spreading on the type-2 plan's
+    interpolation bins contends on LDS atomics. The plan is made at the
+    first backward that needs it, and its points are set from the forward
+    call's whenever they are not its last ones."""
+
+    def __init__(self, n_modes, n_trans, eps, isign, dtype, opts):
+        self._args = (tuple(n_modes), n_trans, eps, isign, dtype, dict(opts))
+        self.plan = None
+        self.key = None
+
+    def __call__(self, g, pts, key):
+        if self.plan is None:
+            n_modes, n_trans, eps, isign, dtype, opts = self._args
+            self.plan = Plan(1, n_modes, n_trans=n_trans, eps=eps, isign=-isign, dtype=dtype,
+                             **opts)
+        if self.key != key:
+            self.plan.setpts(*[p.detach() for p in pts])
+            self.key = key
+        return self.plan.execute(g)
+
+
 class _Type2(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f, plan, *pts):
         c = plan.execute(f)
         ctx.plan = plan
         ctx.pts_gen = plan._pts_gen
+        ctx.fwd_key = next(_FORWARD_IDS)
         ctx.save_for_backward(f, *pts)
         return c
 
@@ -53,7 +85,13 @@ class _Type2(torch.autograd.Function):
     def backward(ctx, g):
         plan, f = _points_of(ctx)
         g = g.contiguous()
-        grad_f = plan.execute_adjoint(g) if ctx.needs_input_grad[0] else None
+        grad_f = None
+        if ctx.needs_input_grad[0]:
+            adj = getattr(plan, "adjoint_spreader", None)
+            if adj is not None:
+                grad_f = adj(g, ctx.saved_tensors[1:], ctx.fwd_key)
+            else:
+                grad_f = plan.execute_adjoint(g)
         grads = [None] * (len(ctx.needs_input_grad) - 2)
         if any(ctx.needs_input_grad[2:]):
             _, dc = plan.interp_grad(f, value=False)
@@ -100,6 +138,7 @@ class NufftType2(torch.nn.Module):
         super().__init__()
         self.plan = Plan(2, tuple(n_modes), n_trans=n_trans, eps=eps, isign=isign, dtype=dtype,
                          **opts)
+        self.plan.adjoint_spreader = _AdjointSpreader(n_modes, n_trans, eps, isign, dtype, opts)
 
     def forward(self, f, *pts):
         self.plan.setpts(*[p.detach() for p in pts])

@@ -72,6 +72,7 @@ BIN_SHAPES = {
 NIN, NHID, NOUT = 11, 32, len(ARMS)
 NPARAM = NHID * NIN + NHID + NHID * NHID + NHID + NOUT * NHID + NOUT
 WEIGHTS = Path(__file__).resolve().parent / "costmodel_weights.json"
+DEFAULT_MARGIN = 0.15  # log units (a predicted gain of 16 %)
 # coarse grid side of the density statistics per dimension (g^dim cells)
 DENSITY_GRID = {1: 4096, 2: 128, 3: 32}
 
@@ -226,6 +227,10 @@ class CostModel:
         self.rejected = rejected or {}
         self.default_misses = default_misses or []
         self.meta = meta or {}
+        # An option other than the defaults is chosen only when its predicted
+        # log time is below the defaults' by more than this margin: smaller
+        # predicted gains are within the network's error on held-out sweeps.
+        self.margin = float(self.meta.get("margin", DEFAULT_MARGIN))
         self._dev_params = None
 
     @classmethod
@@ -265,10 +270,12 @@ class CostModel:
         return problem_class(ttype, dim, tol, dtype) in self.default_misses
 
     def choose(self, ttype, n_modes, M, ntransf, tol, dtype, density, arms=None):
-        """Index into self.arms of the predicted fastest allowed option, or
-        None (the library's heuristic): features outside the training range,
-        a class never swept, or one where the defaults miss tol. `arms`
-        limits the candidates (indices)."""
+        """Index into self.arms of the predicted fastest allowed option (the
+        defaults, 0, unless another is predicted faster by more than
+        self.margin in log time), or None (the library's heuristic):
+        features outside the training range, a class never swept, or one
+        where the defaults miss tol. `arms` limits the candidates
+        (indices)."""
         return self.choose_many([(ttype, n_modes, M, ntransf, tol, dtype, density)], arms)[0]
 
     def choose_many(self, problems, arms=None):
@@ -284,7 +291,10 @@ class CostModel:
             order = [int(i) for i in np.argsort(y)
                      if int(i) not in banned and (arms is None or int(i) in arms)
                      and arm_valid(self.arms[int(i)], ttype, len(n_modes))]
-            out.append(order[0] if order else None)
+            pick = order[0] if order else None
+            if pick not in (None, 0) and 0 in order and y[pick] > y[0] - self.margin:
+                pick = 0  # the defaults, unless the predicted gain exceeds the margin
+            out.append(pick)
         return out
 
     def plan_args(self, choice, ttype, n_modes, tol):

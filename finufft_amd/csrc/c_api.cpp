@@ -359,7 +359,8 @@ int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout, double
       if (dout) {
         dout[0] = p->beta();
         dout[1] = p->upsampfac();
-        dout[2] = dout[3] = 0;
+        dout[2] = p->od_np();
+        dout[3] = 0;
       }
     };
     void *q = const_cast<void *>(plan);

@@ -250,6 +250,7 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
   user_bin_[1] = opts_.gpu_binsizey;
   user_bin_[2] = opts_.gpu_binsizez;
   if (opts_.gpu_method != 3) opts_.gpu_np = 0;
+  user_np_ = opts_.gpu_np;  // as the bins: chosen afresh at every type-3 setpts
   if (type_ != 3) {
     for (int d = 0; d < 3; ++d) ms_[d] = d < dim_ ? nmodes[d] : 1;
     spread_only_ = opts_.gpu_spreadinterponly != 0;
@@ -267,8 +268,12 @@ Plan<T>::Plan(int type, int dim, const int *nmodes, int iflag, int ntransf, T to
     fail(FINUFFT_ERR_INSUFFICIENT_SHMEM, "invalid gpu_method");
   if (opts_.gpu_method == 4)
     fail(FINUFFT_ERR_METHOD_NOTVALID, "block-gather (gpu_method=4) is not provided");
-  if (opts_.gpu_method == 3)
-    fail(FINUFFT_ERR_METHOD_NOTVALID, "output-driven spreading (gpu_method=3) is not provided");
+  // The output-driven method only spreads: the reference's interpolation
+  // dispatch refuses it at execute (spreadinterp.cpp:53-61), this plan at
+  // makeplan with the same code. (The adjoint of a method-3 type-1 plan
+  // interpolates by the LDS method on the plan's bins.)
+  if (opts_.gpu_method == 3 && type_ == 2)
+    fail(FINUFFT_ERR_METHOD_NOTVALID, "gpu_method 3 (output driven) only spreads");
 
   device_guard dg(opts_.gpu_device_id);
   upload_coefficients();
@@ -388,10 +393,14 @@ void Plan<T>::setup_bins() {
   };
   geom_.colS = 0;
   geom_.bank = 0;
+  od_np_ = 0;
+  lds_od_ = 0;
+  // Output-driven spreading (method 3; plans of types 1 and 3).
+  const bool od = opts_.gpu_method == 3 && type_ != 2;
   // Column spreading: 3-D fp32 spreading plans with ns <= 8. User bin sizes
   // select it when x and y are 8 (the footprint of one wave) and z is at most
   // COL_ZC.
-  const bool col_ok = opts_.gpu_method != 1 && type_ != 2 && dim_ == 3 &&
+  const bool col_ok = !od && opts_.gpu_method != 1 && type_ != 2 && dim_ == 3 &&
                       std::is_same<T, float>::value && ns_ <= COL_MAX_NS && nf_[0] >= 8 &&
                       nf_[1] >= 8 &&
                       (!user_bins ||
@@ -416,7 +425,41 @@ void Plan<T>::setup_bins() {
       lds_bytes_ = size_t(tile_cells(3, bs, ns_)) * 2 * sizeof(T);
     }
   }
-  if (col_done) {
+  if (od) {
+    // Start-node bins whose fp64 tile and np staged points fit the LDS: the
+    // user's np and bins, the user's bins with as many points as fit (a
+    // multiple of 16, at most 256), or 64 points (or the user's np) with the
+    // largest bins that fit (the cases of heuristics.cu:394-423).
+    const size_t cell = 2 * sizeof(double);
+    int np = user_np_;
+    bool ok = np >= 0;
+    if (ok && user_bins) {
+      for (int d = 0; d < 3; ++d)
+        bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : std::min(8, nf_[d])) : 1;
+      const int64_t tile = tile_cells(dim_, bs, ns_) * int64_t(cell);
+      if (np == 0) {
+        const int64_t room = (int64_t(lds_limit) - tile) /
+                             int64_t(od_stage_bytes(dim_, ns_, 1, sizeof(T)));
+        np = int(std::min<int64_t>(256, std::max<int64_t>(room, 0)) & ~int64_t(15));
+      }
+      ok = np >= 1 && tile + int64_t(od_stage_bytes(dim_, ns_, np, sizeof(T))) <= lds_limit;
+    } else if (ok) {
+      if (np == 0) np = 64;
+      const int64_t stage = int64_t(od_stage_bytes(dim_, ns_, np, sizeof(T)));
+      const int64_t min_units = (1024 + batchsize_ - 1) / batchsize_;
+      ok = stage < lds_limit &&
+           choose_bins(dim_, nf_, ns_, cell, size_t(lds_limit - stage), false, min_units, bs);
+    }
+    if (!ok)
+      fail(FINUFFT_ERR_INSUFFICIENT_SHMEM,
+           "output-driven spreading: the tile and np staged points do not fit the LDS");
+    geom_.start_bins = 1;
+    colored_ = false;
+    lds_spread_ = size_t(tile_cells(dim_, bs, ns_)) * cell;
+    lds_bytes_ = size_t(tile_cells(dim_, bs, ns_)) * 2 * sizeof(T);
+    od_np_ = np;
+    lds_od_ = lds_spread_ + od_stage_bytes(dim_, ns_, np, sizeof(T));
+  } else if (col_done) {
     // bins chosen above
   } else if (opts_.gpu_method == 1) {
     set_ref_bins();
@@ -512,19 +555,15 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
   if (opts_.gpu_method == 1 && !opts_.gpu_sort) {
     iota(idx_.get(), M_, stream_);
   } else {
-    bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_,
-                geom_.colS ? colstart_.get() : nullptr);
-    // Coordinates in sorted order, so that spread/interp read them coalesced.
+    // the permutation, and the coordinates in sorted order so that
+    // spread/interp read them coalesced
+    T *sorted[3] = {nullptr, nullptr, nullptr};
     for (int d = 0; d < dim_; ++d) {
       sxyz_[d].reserve_at_least(std::max(M_, 1));
-      permute<T>(idx_.get(), pts[d], sxyz_[d].get(), M_, stream_);
+      sorted[d] = sxyz_[d].get();
     }
-  }
-  // Spreading with the LDS kernels reads the strengths in the sorted order.
-  sorted_c_ = type_ != 2 && opts_.gpu_method != 1 && !spread_gm_;
-  if (sorted_c_) {
-    inv_.reserve_at_least(std::max(M_, 1));
-    inverse_perm(idx_.get(), inv_.get(), M_, stream_);
+    bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_,
+                geom_.colS ? colstart_.get() : nullptr, sorted);
   }
   if (opts_.gpu_method != 1) {
     substart_.reserve_at_least(size_t(nb) + 1);
@@ -668,7 +707,7 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   SpreadCtx<T> c;
   c.dim = dim_;
   c.ns = ns_;
-  c.method = opts_.gpu_method == 1 ? 1 : 2;
+  c.method = opts_.gpu_method == 1 ? 1 : (opts_.gpu_method == 3 && type_ != 2 ? 3 : 2);
   c.M = M_;
   const bool sorted = !(opts_.gpu_method == 1 && !opts_.gpu_sort);
   for (int d = 0; d < 3; ++d) {
@@ -698,7 +737,8 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.colstart = colstart_.get();
   c.dense_min = dense_min_;
   c.n_dense_sub = n_dense_sub_;
-  c.c_sorted = sorted_c_ ? 1 : 0;
+  c.od_np = od_np_;
+  c.lds_od = lds_od_;
   return c;
 }
 
@@ -719,11 +759,6 @@ void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
     zero<T>(fw, nftot_ * blk, stream_);
     timer_.end(stream_);
     timer_.begin(ST_SPREAD, stream_);
-    if (sorted_c_) {
-      csort_.reserve_at_least(size_t(M_) * blk);
-      scatter_sorted<T>(c, nullptr, inv_.get(), csort_.get(), M_, blk, stream_);
-      c = csort_.get();
-    }
     spread<T>(ctx, c, fw, blk, stream_);
     timer_.end(stream_);
     if (spread_only_) continue;
@@ -773,10 +808,7 @@ void Plan<T>::exec_type3(C *d_c, C *d_fk) {
     const int blk = std::min(ntransf_ - i * batchsize_, batchsize_);
     const C *c = d_c + int64_t(i) * batchsize_ * M_;
     C *fk = d_fk + int64_t(i) * batchsize_ * Nk_;
-    if (sorted_c_)  // the prephase applied while scattering into the sorted order
-      scatter_sorted<T>(c, prephase_.get(), inv_.get(), cpre_.get(), M_, blk, stream_);
-    else
-      cmul<T>(c, prephase_.get(), cpre_.get(), M_, blk, stream_);
+    cmul<T>(c, prephase_.get(), cpre_.get(), M_, blk, stream_);
     zero<T>(fw_.get(), nftot_ * blk, stream_);
     spread<T>(ctx, cpre_.get(), fw_.get(), blk, stream_);
     inner_->exec_type2(fk, fw_.get(), inner_->iflag_, blk);

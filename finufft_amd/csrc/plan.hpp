@@ -60,6 +60,7 @@ class Plan {
   int kernel_formula() const { return formula_; }
   double upsampfac() const { return opts_.upsampfac; }
   int method() const { return opts_.gpu_method; }
+  int od_np() const { return od_np_; }
   int nf(int d) const { return nf_[d]; }
   int binsize(int d) const { return geom_.binsize[d]; }
   int batchsize() const { return batchsize_; }
@@ -122,12 +123,6 @@ class Plan {
   // sort / subproblems
   BinGeom geom_;
   dbuf<int> idx_, binstart_, substart_, sub2bin_, colstart_;
-  // Spreading plans of the LDS methods: the inverse sort permutation, and
-  // the strengths of an execute scattered into the sorted order through it
-  // (the kernels then read them coalesced instead of gathering c[idx[q]]).
-  dbuf<int> inv_;
-  dbuf<typename cplx_of<T>::type> csort_;
-  bool sorted_c_ = false;
   dbuf<T> sxyz_[3];  // coordinates in sorted order
   bool colored_ = false;
   int sub_capacity_ = 0;
@@ -138,6 +133,9 @@ class Plan {
   size_t lds_bytes_ = 0;   // interpolation tile
   size_t lds_spread_ = 0;  // spreading tile (fp64) plus staging
   bool spread_gm_ = false;  // spreading tile too large: point-driven spreading
+  int user_np_ = 0;         // the caller's gpu_np (method 3)
+  int od_np_ = 0;           // output-driven spreading: points staged per batch
+  size_t lds_od_ = 0;       // its LDS bytes (fp64 tile + staging)
 
   // fine grid work buffer and FFT
   dbuf<C> fw_;

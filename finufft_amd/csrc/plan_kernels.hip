@@ -297,49 +297,7 @@ __global__ void cmul_kernel(const cplx_t<T> *a, const cplx_t<T> *b, cplx_t<T> *o
   }
 }
 
-// out[t*n + inv[i]] = a[t*n + i] (* b[i]): a strength array in the sorted
-// order, read in the caller's order (coalesced) and written through the
-// inverse sort permutation.
-template <typename T, bool MUL>
-__global__ void scatter_sorted_kernel(const cplx_t<T> *__restrict__ a,
-                                      const cplx_t<T> *__restrict__ b, const int *__restrict__ inv,
-                                      cplx_t<T> *__restrict__ out, int n) {
-  const int64_t t = blockIdx.y;
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    cplx_t<T> u = a[t * n + i];
-    if (MUL) {
-      const cplx_t<T> v = b[i];
-      u = cplx_t<T>{u.x * v.x - u.y * v.y, u.x * v.y + u.y * v.x};
-    }
-    out[t * n + inv[i]] = u;
-  }
-}
-
-__global__ void inverse_perm_kernel(const int *__restrict__ idx, int *__restrict__ inv, int n) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    inv[idx[i]] = int(i);
-}
-
 }  // namespace
-
-void inverse_perm(const int *idx, int *inv, int n, hipStream_t stream) {
-  if (n <= 0) return;
-  inverse_perm_kernel<<<ew_grid(n), EW_T, 0, stream>>>(idx, inv, n);
-  launch_check("inverse_perm_kernel");
-}
-
-template <typename T>
-void scatter_sorted(const cplx_t<T> *a, const cplx_t<T> *b, const int *inv, cplx_t<T> *out,
-                    int n, int blk, hipStream_t stream) {
-  if (n <= 0 || blk <= 0) return;
-  if (b)
-    scatter_sorted_kernel<T, true><<<ew_grid(n, blk), EW_T, 0, stream>>>(a, b, inv, out, n);
-  else
-    scatter_sorted_kernel<T, false><<<ew_grid(n, blk), EW_T, 0, stream>>>(a, b, inv, out, n);
-  launch_check("scatter_sorted_kernel");
-}
 
 template <typename T>
 void fseries(int dim, const int nf[3], int q, const double *f, const double *z, T *const out[3],
@@ -471,9 +429,7 @@ void cmul(const cplx_t<T> *a, const cplx_t<T> *b, cplx_t<T> *out, int n, int blk
                              const double[3], int, int, const double *, const double *, int,   \
                              cplx_t<T> *, hipStream_t);                                        \
   template void cmul<T>(const cplx_t<T> *, const cplx_t<T> *, cplx_t<T> *, int, int,           \
-                        hipStream_t);                                                          \
-  template void scatter_sorted<T>(const cplx_t<T> *, const cplx_t<T> *, const int *,            \
-                                  cplx_t<T> *, int, int, hipStream_t);
+                        hipStream_t);
 FA_INST(float)
 FA_INST(double)
 #undef FA_INST

@@ -78,15 +78,5 @@ void t3_deconv(int dim, const T *const stu[3], const T *const stup[3], const dou
 template <typename T>
 void cmul(const cplx_t<T> *a, const cplx_t<T> *b, cplx_t<T> *out, int n, int blk,
           hipStream_t stream);
-// inv[idx[i]] = i for the sort permutation idx of n points.
-void inverse_perm(const int *idx, int *inv, int n, hipStream_t stream);
-// out[t*n + inv[i]] = a[t*n + i] (times b[i] when b is not null), t < blk:
-// strengths in the sort order. Reads are in the caller's order (coalesced)
-// and writes go through inv: scattered 8- or 16-byte writes cost a quarter
-// of the traffic of the same gather (partial-line writes against full-line
-// reads).
-template <typename T>
-void scatter_sorted(const cplx_t<T> *a, const cplx_t<T> *b, const int *inv, cplx_t<T> *out,
-                    int n, int blk, hipStream_t stream);
 
 }  // namespace fa

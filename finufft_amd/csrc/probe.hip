@@ -32,8 +32,31 @@ enum ProbeMode {
   P_FMA_F32 = 8,      // v_fma_f32 (VALU issue rate)
   P_ADD_F32_ROT = 9,  // ds_add_f32, each lane cycling through 64 addresses
   P_ADD_F64_ROT = 10, // ds_add_f64, each lane cycling through 32 addresses
-  P_COUNT = 11
+  // ds_read_b64 with lane l of a wave at qword l * S (+ a wave-uniform
+  // shift): bank conflicts appear at the strides that map several lanes of
+  // one issue group to one bank
+  P_READ_B64_S1 = 11,
+  P_READ_B64_S2 = 12,
+  P_READ_B64_S16 = 13,
+  P_READ_B64_S32 = 14,
+  P_READ_B64_S64 = 15,
+  // ds_add_f32 as P_ADD_F32, with the wave's fp32 denormals flushed (MODE
+  // register FP_DENORM bits [5:4] cleared at kernel start; the library's
+  // kernels run with fp32 denormals kept)
+  P_ADD_F32_FTZ = 16,
+  P_COUNT = 17
 };
+
+// s_setreg operand: hardware register MODE (1), bit offset 4, 2 bits.
+constexpr int HWREG_MODE_FP32_DENORM = ((2 - 1) << 11) | (4 << 6) | 1;
+
+constexpr int b64_stride(int mode) {
+  return mode == P_READ_B64_S1    ? 1
+         : mode == P_READ_B64_S2  ? 2
+         : mode == P_READ_B64_S16 ? 16
+         : mode == P_READ_B64_S32 ? 32
+                                  : 64;
+}
 
 template <int MODE>
 __global__ void __launch_bounds__(PROBE_T) lds_probe_kernel(float *out, int reps) {
@@ -43,7 +66,12 @@ __global__ void __launch_bounds__(PROBE_T) lds_probe_kernel(float *out, int reps
   for (int i = tid; i < PROBE_WORDS; i += PROBE_T) t[i] = float(i & 7);
   __syncthreads();
   float acc = 0.f;
-  if constexpr (MODE == P_ADD_F32) {
+  if constexpr (MODE == P_ADD_F32_FTZ) {
+    __builtin_amdgcn_s_setreg(HWREG_MODE_FP32_DENORM, 0);
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) atomicAdd(&t[(tid + k * PROBE_T) & (PROBE_WORDS - 1)], 1.0f);
+  } else if constexpr (MODE == P_ADD_F32) {
     for (int r = 0; r < reps; ++r)
 #pragma unroll
       for (int k = 0; k < 8; ++k) atomicAdd(&t[(tid + k * PROBE_T) & (PROBE_WORDS - 1)], 1.0f);
@@ -82,6 +110,17 @@ __global__ void __launch_bounds__(PROBE_T) lds_probe_kernel(float *out, int reps
       for (int k = 0; k < 8; ++k) {
         const int a = MODE == P_READ_B32 ? tid + k * PROBE_T : wave * 64 + k * 8 + (r & 7);
         acc += t[(a + r) & (PROBE_WORDS - 1)];
+      }
+  } else if constexpr (MODE >= P_READ_B64_S1 && MODE <= P_READ_B64_S64) {
+    // PROBE_WORDS / 2 qwords; lane * 64 + k + r stays below it after the mask
+    const double *v = reinterpret_cast<const double *>(t);
+    const int lane = tid & 63;
+    for (int r = 0; r < reps; ++r)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const double q = v[(lane * b64_stride(MODE) + k + (r & 7) + wave * 8) &
+                           (PROBE_WORDS / 2 - 1)];
+        acc += float(q);
       }
   } else if constexpr (MODE == P_READ_B128 || MODE == P_READ_B128_U) {
     const float4 *v = reinterpret_cast<const float4 *>(t);
@@ -138,6 +177,12 @@ void lds_probe(int mode, int blocks, int reps, double *ms, double *wave_ops) {
       case P_READ_B128_U: return launch_probe<P_READ_B128_U>(out.get(), blocks, r, nullptr);
       case P_ADD_F32_ROT: return launch_probe<P_ADD_F32_ROT>(out.get(), blocks, r, nullptr);
       case P_ADD_F64_ROT: return launch_probe<P_ADD_F64_ROT>(out.get(), blocks, r, nullptr);
+      case P_READ_B64_S1: return launch_probe<P_READ_B64_S1>(out.get(), blocks, r, nullptr);
+      case P_READ_B64_S2: return launch_probe<P_READ_B64_S2>(out.get(), blocks, r, nullptr);
+      case P_READ_B64_S16: return launch_probe<P_READ_B64_S16>(out.get(), blocks, r, nullptr);
+      case P_READ_B64_S32: return launch_probe<P_READ_B64_S32>(out.get(), blocks, r, nullptr);
+      case P_READ_B64_S64: return launch_probe<P_READ_B64_S64>(out.get(), blocks, r, nullptr);
+      case P_ADD_F32_FTZ: return launch_probe<P_ADD_F32_FTZ>(out.get(), blocks, r, nullptr);
       default: return launch_probe<P_FMA_F32>(out.get(), blocks, r, nullptr);
     }
   };

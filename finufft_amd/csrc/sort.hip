@@ -14,6 +14,8 @@
 // order. Bin start offsets are read off the sorted keys.
 #include "sort.hpp"
 
+#include <type_traits>
+
 #include "kernel_math.hpp"
 
 namespace fa {
@@ -190,18 +192,40 @@ __device__ __forceinline__ T folded(T x, int n) {
   return (r >= T(0) && r < T(n)) ? r : T(0);
 }
 
+// A point's coordinates packed as one aligned vector of pack_stride(DIM)
+// values ((x, y) in 2-D, (x, y, z, 0) in 3-D), so that the sorted copies
+// cost one gathered read per point instead of one per coordinate (a random
+// read costs a whole cache line, however few of its bytes are used).
+template <typename T, int DIM>
+__device__ __forceinline__ void pack_point(T *__restrict__ p, const T (&v)[3]) {
+  using V2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+  if constexpr (DIM == 2) {
+    *reinterpret_cast<V2 *>(p) = V2{v[0], v[1]};
+  } else if constexpr (DIM == 3) {
+    if constexpr (std::is_same<T, float>::value) {
+      *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], 0.f);
+    } else {
+      reinterpret_cast<V2 *>(p)[0] = V2{v[0], v[1]};
+      reinterpret_cast<V2 *>(p)[1] = V2{v[2], T(0)};
+    }
+  }
+}
+
 template <typename T, int DIM>
 __global__ void bin_key_kernel(int M, const T *__restrict__ x, const T *__restrict__ y,
                                const T *__restrict__ z, BinGeom g, T inv0, T inv1, T inv2,
-                               int *__restrict__ keys) {
+                               int *__restrict__ keys, T *__restrict__ packed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
+  T v[3] = {x[i], T(0), T(0)};
+  if constexpr (DIM > 1) v[1] = y[i];
+  if constexpr (DIM > 2) v[2] = z[i];
+  if (DIM > 1 && packed) pack_point<T, DIM>(packed + int64_t(i) * pack_stride(DIM), v);
   if (g.start_bins) {
-    const T *c[3] = {x, y, z};
     int key = 0, stride = 1, l[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      int s = start_index(c[d][i], g.nf[d], g.ns, g.fold64);
+      int s = start_index(v[d], g.nf[d], g.ns, g.fold64);
       s = s < 0 ? s + g.nf[d] : s;
       const int b = min(max(s / g.binsize[d], 0), g.nbins[d] - 1);
       l[d] = min(max(s - b * g.binsize[d], 0), g.binsize[d] - 1);
@@ -220,11 +244,40 @@ __global__ void bin_key_kernel(int M, const T *__restrict__ x, const T *__restri
     keys[i] = key;
     return;
   }
-  int key = bin_of(folded(x[i], g.nf[0]), inv0, g.nbins[0]);
-  if constexpr (DIM > 1) key += g.nbins[0] * bin_of(folded(y[i], g.nf[1]), inv1, g.nbins[1]);
+  int key = bin_of(folded(v[0], g.nf[0]), inv0, g.nbins[0]);
+  if constexpr (DIM > 1) key += g.nbins[0] * bin_of(folded(v[1], g.nf[1]), inv1, g.nbins[1]);
   if constexpr (DIM > 2)
-    key += g.nbins[0] * g.nbins[1] * bin_of(folded(z[i], g.nf[2]), inv2, g.nbins[2]);
+    key += g.nbins[0] * g.nbins[1] * bin_of(folded(v[2], g.nf[2]), inv2, g.nbins[2]);
   keys[i] = key;
+}
+
+// out_d[q] = coordinate d of point idx[q], from the packed points.
+template <typename T, int DIM>
+__global__ void gather_points_kernel(const int *__restrict__ idx, const T *__restrict__ packed,
+                                     T *__restrict__ o0, T *__restrict__ o1, T *__restrict__ o2,
+                                     int M) {
+  using V2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= M) return;
+  const T *p = packed + int64_t(idx[q]) * pack_stride(DIM);
+  if constexpr (DIM == 2) {
+    const V2 v = *reinterpret_cast<const V2 *>(p);
+    o0[q] = v.x;
+    o1[q] = v.y;
+  } else {
+    if constexpr (std::is_same<T, float>::value) {
+      const float4 v = *reinterpret_cast<const float4 *>(p);
+      o0[q] = v.x;
+      o1[q] = v.y;
+      o2[q] = v.z;
+    } else {
+      const V2 a = reinterpret_cast<const V2 *>(p)[0];
+      const V2 b = reinterpret_cast<const V2 *>(p)[1];
+      o0[q] = a.x;
+      o1[q] = a.y;
+      o2[q] = b.x;
+    }
+  }
 }
 
 // binstart[g] = first sorted position whose key / div is >= g, g in [0, ng].
@@ -306,7 +359,7 @@ void iota(int *idx, int M, hipStream_t stream) {
 
 template <typename T>
 void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *binstart,
-              SortScratch &s, hipStream_t stream, int *colstart) {
+              SortScratch &s, hipStream_t stream, int *colstart, T *const sorted[3]) {
   const int nb = g.total();
   const bool cols = g.colS > 0 && g.dim == 3 && g.start_bins;
   const bool banks = !cols && g.bank && g.start_bins;
@@ -315,6 +368,8 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
   const int nsub = cols ? g.nsub() : 1;
   if (M == 0 || (nb == 1 && !cols && !banks)) {
     iota(idx, M, stream);
+    if (sorted)
+      for (int d = 0; d < g.dim; ++d) permute<T>(idx, xyz[d], sorted[d], M, stream);
     two_point_bounds_kernel<<<blocks_for(nb + 1, 256), 256, 0, stream>>>(binstart, nb, M);
     launch_check("two_point_bounds_kernel");
     if (cols) {
@@ -327,20 +382,27 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
   s.keys_a.reserve_at_least(M);
   s.keys_b.reserve_at_least(M);
   s.vals_b.reserve_at_least(M);
+  // 2-D and 3-D: the key pass also packs each point's coordinates, gathered
+  // once per point into the sorted copies after the last radix pass
+  T *packed = nullptr;
+  if (sorted && g.dim > 1) {
+    s.packed.reserve_at_least(size_t(M) * pack_stride(g.dim) * sizeof(T));
+    packed = reinterpret_cast<T *>(s.packed.get());
+  }
   const T inv0 = T(1) / T(g.binsize[0]), inv1 = T(1) / T(g.binsize[1]),
           inv2 = T(1) / T(g.binsize[2]);
   switch (g.dim) {
     case 1:
       bin_key_kernel<T, 1><<<blocks_for(M, 256), 256, 0, stream>>>(
-          M, xyz[0], xyz[1], xyz[2], g, inv0, inv1, inv2, s.keys_a.get());
+          M, xyz[0], xyz[1], xyz[2], g, inv0, inv1, inv2, s.keys_a.get(), nullptr);
       break;
     case 2:
       bin_key_kernel<T, 2><<<blocks_for(M, 256), 256, 0, stream>>>(
-          M, xyz[0], xyz[1], xyz[2], g, inv0, inv1, inv2, s.keys_a.get());
+          M, xyz[0], xyz[1], xyz[2], g, inv0, inv1, inv2, s.keys_a.get(), packed);
       break;
     default:
       bin_key_kernel<T, 3><<<blocks_for(M, 256), 256, 0, stream>>>(
-          M, xyz[0], xyz[1], xyz[2], g, inv0, inv1, inv2, s.keys_a.get());
+          M, xyz[0], xyz[1], xyz[2], g, inv0, inv1, inv2, s.keys_a.get(), packed);
   }
   launch_check("bin_key_kernel");
 
@@ -380,12 +442,25 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
         kin, M, nb, banks ? 32 : 1, binstart);
     launch_check("bin_bounds_kernel");
   }
+  if (sorted) {
+    if (g.dim == 1) {
+      permute<T>(idx, xyz[0], sorted[0], M, stream);
+    } else if (g.dim == 2) {
+      gather_points_kernel<T, 2><<<blocks_for(M, 256), 256, 0, stream>>>(
+          idx, packed, sorted[0], sorted[1], nullptr, M);
+      launch_check("gather_points_kernel");
+    } else {
+      gather_points_kernel<T, 3><<<blocks_for(M, 256), 256, 0, stream>>>(
+          idx, packed, sorted[0], sorted[1], sorted[2], M);
+      launch_check("gather_points_kernel");
+    }
+  }
 }
 
 template void bin_sort<float>(const BinGeom &, int, const float *const[3], int *, int *,
-                              SortScratch &, hipStream_t, int *);
+                              SortScratch &, hipStream_t, int *, float *const[3]);
 template void bin_sort<double>(const BinGeom &, int, const double *const[3], int *, int *,
-                               SortScratch &, hipStream_t, int *);
+                               SortScratch &, hipStream_t, int *, double *const[3]);
 
 void build_subproblems(const int *binstart, int nb, int maxsub, int dense_min, int *substart,
                        int *sub2bin, SortScratch &s, hipStream_t stream) {

@@ -46,18 +46,25 @@ struct BinGeom {
   }
 };
 
-// Scratch reused across setpts calls (grows only).
+// Values per point of the packed coordinates the sort gathers from.
+__host__ __device__ constexpr int pack_stride(int dim) { return dim == 3 ? 4 : dim; }
+
+// Scratch reused across setpts calls (grows only). `packed` holds the points'
+// coordinates packed per point (pack_stride(dim) values of the plan's real
+// type) while sorted copies are made.
 struct SortScratch {
   dbuf<int> keys_a, keys_b, vals_b, hist, scan_tmp;
+  dbuf<unsigned char> packed;
 };
 
 // idx[M]: sorted point order. binstart[nbins+1]: first sorted position of each
 // bin, binstart[nbins] = M. With g.colS, colstart[nbins * nsub + 1] receives
-// the first sorted position of each (bin, sub-column). Nothing is
+// the first sorted position of each (bin, sub-column). sorted[d] (M values
+// each, d < dim; or null): the coordinates in the sorted order. Nothing is
 // synchronized with the host.
 template <typename T>
 void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *binstart,
-              SortScratch &s, hipStream_t stream, int *colstart = nullptr);
+              SortScratch &s, hipStream_t stream, int *colstart, T *const sorted[3]);
 
 // idx[i] = i.
 void iota(int *idx, int M, hipStream_t stream);

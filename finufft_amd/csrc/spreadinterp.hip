@@ -382,13 +382,13 @@ __device__ __forceinline__ void spread_into_tile(const SpreadCtx<T> &ctx,
   for (int i = threadIdx.x; i < st.npts; i += BIN_THREADS) {
     const int q = st.first + i;
     const int j = ctx.idx[q];
-    const cplx_t<T> cv = c[ctx.c_sorted ? q : j];
     T ker[DIM][NS];
     int l[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
       l[d] = st.local_start(start_and_kernel<T, NS>(coord(ctx, d, q, j), ctx.nf[d], ker[d], ctx.kp),
                             d, ctx.nf[d]);
+    const cplx_t<T> cv = c[j];
     if constexpr (DIM == 1) {
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
@@ -486,6 +486,89 @@ __global__ void __launch_bounds__(BIN_THREADS)
   __syncthreads();
   tile_to_grid<DIM, BIN_THREADS, false>(ctx, st.P, st.off, st.ntile, re, im,
                                         fw + int64_t(t) * nftot);
+}
+
+// ------------------------------------------------- output-driven spreading
+//
+// gpu_method 3, the reference's output-driven spreader
+// (src/cuda/spread_output_driven_inst.cu:17-103): a subproblem's points are
+// staged np at a time (each point's kernel values, first tile cell and
+// strength in LDS); then the workgroup takes the staged points one after
+// another, its threads splitting the point's ns^d support cells, so that no
+// two threads update one cell at once and the fp64 tile takes plain
+// read-modify-writes; a barrier separates consecutive points. The tile is
+// added to the grid with atomics, as for subproblems.
+//
+// LDS: the fp64 tile (re, im planes of ntile cells), then np strengths (re
+// and im arrays of T), np * DIM * NS kernel values (T) and np * DIM first
+// cells (int); od_stage_bytes gives the staging part.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(BIN_THREADS)
+    spread_od_kernel(SpreadCtx<T> ctx, const cplx_t<T> *__restrict__ c,
+                     cplx_t<T> *__restrict__ fw, int64_t nftot, int np) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *re = reinterpret_cast<double *>(lds_raw);
+  StartTile<DIM, NS> st;
+  if (!st.init_sub(ctx, blockIdx.x) || st.npts == 0) return;
+  double *im = re + st.ntile;
+  T *s_re = reinterpret_cast<T *>(im + st.ntile);
+  T *s_im = s_re + np;
+  T *s_ker = s_im + np;
+  int *s_l = reinterpret_cast<int *>(s_ker + np * DIM * NS);
+  const cplx_t<T> *cv = c + int64_t(blockIdx.y) * ctx.M;
+  for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
+    re[n] = 0.0;
+    im[n] = 0.0;
+  }
+  constexpr int NSD = DIM == 1 ? NS : DIM == 2 ? NS * NS : NS * NS * NS;
+  for (int b0 = 0; b0 < st.npts; b0 += np) {
+    const int nb = min(np, st.npts - b0);
+    // the previous batch's last point was followed by a barrier (and the
+    // tile clearing by the one below), so the staging arrays are free
+    for (int i = threadIdx.x; i < nb; i += BIN_THREADS) {
+      const int q = st.first + b0 + i;
+      const int j = ctx.idx[q];
+      T ker[DIM][NS];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        s_l[i * DIM + d] = st.local_start(
+            start_and_kernel<T, NS>(coord(ctx, d, q, j), ctx.nf[d], ker[d], ctx.kp), d,
+            ctx.nf[d]);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) s_ker[(i * DIM + d) * NS + a] = ker[d][a];
+      }
+      const cplx_t<T> v = cv[j];
+      s_re[i] = v.x;
+      s_im[i] = v.y;
+    }
+    __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+      const T cr = s_re[i], ci = s_im[i];
+      const T *k = s_ker + i * DIM * NS;
+      const int *l = s_l + i * DIM;
+      for (int s = threadIdx.x; s < NSD; s += BIN_THREADS) {
+        const int a = s % NS;
+        T w = k[a];
+        int cell = l[0] + a;
+        if constexpr (DIM > 1) {
+          const int b = (s / NS) % NS;
+          w *= k[NS + b];
+          cell += (l[1] + b) * st.P[0];
+        }
+        if constexpr (DIM > 2) {
+          const int e = s / (NS * NS);
+          w *= k[2 * NS + e];
+          cell += (l[2] + e) * st.P[0] * st.P[1];
+        }
+        re[cell] += double(cr * w);
+        im[cell] += double(ci * w);
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  tile_to_grid<DIM, BIN_THREADS, false>(ctx, st.P, st.off, st.ntile, re, im,
+                                        fw + int64_t(blockIdx.y) * nftot);
 }
 
 
@@ -615,9 +698,8 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
   // (clearing the tile), and each next batch's strengths while the current
   // batch is walked, so that memory latency overlaps work.
   fetch(u0 + lane);
-  const bool csort = ctx.c_sorted != 0;
   float2 fc = make_float2(0.f, 0.f);
-  if (u0 + lane < u1) fc = c[csort ? u0 + lane : fj];
+  if (u0 + lane < u1) fc = c[fj];
   start();
 
   float ar[ZD], ai[ZD];
@@ -632,9 +714,6 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
     const float x = fx, y = fy, z = fz;
     const float2 cv = fc;
     fetch(p + 64 + lane);
-    // the next batch's strengths: at once when they are in the sorted order,
-    // else after the staging below (they need the next sorted index)
-    if (csort && p + 64 + lane < u1) fc = c[p + 64 + lane];
     int posv = 0;  // lz + 64 * sub-column of this lane's point
     if (lane < n) {
       float kx[NS], ky[NS], kz[NS];
@@ -678,7 +757,7 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
       r[6] = make_float4(wy8[0], wy8[1], wy8[2], wy8[3]);
       r[7] = make_float4(wy8[4], wy8[5], wy8[6], wy8[7]);
     }
-    if (!csort && p + 64 + lane < u1) fc = c[fj];
+    if (p + 64 + lane < u1) fc = c[fj];  // the next batch's strengths
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -1152,6 +1231,13 @@ void spread_ns(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int b
     dim3 grid((ctx.M + SPREAD_THREADS - 1) / SPREAD_THREADS, blk);
     spread_gm_kernel<T, DIM, NS><<<grid, SPREAD_THREADS, 0, stream>>>(ctx, c, fw, nftot);
     launch_check("spread_gm_kernel");
+  } else if (ctx.method == 3) {
+    if (ctx.od_np < 1) fail(FINUFFT_ERR_INVALID_ARGUMENT, "output-driven spreading: np < 1");
+    auto k = spread_od_kernel<T, DIM, NS>;
+    allow_lds(k, ctx.lds_od);
+    k<<<dim3(ctx.sub_capacity, blk), BIN_THREADS, ctx.lds_od, stream>>>(ctx, c, fw, nftot,
+                                                                        ctx.od_np);
+    launch_check("spread_od_kernel");
   } else if (ctx.colS) {
     if constexpr (DIM == 3 && std::is_same<T, float>::value && NS <= COL_MAX_NS) {
       if (ctx.colored) {

@@ -28,7 +28,9 @@ template <typename T>
 struct SpreadCtx {
   int dim = 1;
   int ns = 2;
-  int method = 1;  // 1: point driven (GM / GM-sort), 2: LDS subproblems (SM)
+  // 1: point driven (GM / GM-sort), 2: LDS subproblems (SM), 3: output
+  // driven (spreading only; LDS subproblems, np points staged at a time)
+  int method = 1;
   int M = 0;
   int nf[3] = {1, 1, 1};
   const T *xyz[3] = {nullptr, nullptr, nullptr};
@@ -71,12 +73,21 @@ struct SpreadCtx {
 
   int colS = 0;
   const int *colstart = nullptr;
-  // Spreading strengths already in the sorted order (c[q] for the q-th
-  // sorted point rather than c[idx[q]]): LDS spreading kernels only.
-  int c_sorted = 0;
   int dense_min = 0;
   int n_dense_sub = 0;
+
+  // Output-driven spreading (method 3): points staged per batch and the
+  // kernel's LDS bytes (fp64 tile + od_stage_bytes).
+  int od_np = 0;
+  size_t lds_od = 0;
 };
+
+// Staging bytes of the output-driven spreader for np points: strengths (re
+// and im arrays), kernel values (dim * ns per point) and first tile cells
+// (dim ints per point).
+inline constexpr size_t od_stage_bytes(int dim, int ns, int np, size_t real_bytes) {
+  return size_t(np) * ((2 + size_t(dim) * ns) * real_bytes + size_t(dim) * sizeof(int));
+}
 
 // Global-memory pointer types. A pointer read from memory (a work
 // descriptor) is generic to the compiler, which then uses flat instructions

@@ -118,7 +118,8 @@ FINUFFT_EXPORT int finufft_amd_density_stats_noalloc(int dim, int64_t M, const v
  *           batchsize, M, number of bins, sub-column side of the column-window
  *           spreader (0 when unused), whole-bin parity-colored spreading (0/1),
  *           number of dense-bin subproblems.
- * dout[4]:  beta, upsampfac, 0, 0. */
+ * dout[4]:  beta, upsampfac, points staged per batch by output-driven
+ *           spreading (gpu_method 3; else 0), 0. */
 FINUFFT_EXPORT int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout,
                                          double *dout);
 

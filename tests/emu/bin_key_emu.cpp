@@ -1,10 +1,14 @@
 // Runs the sort's bin-key kernel (finufft_amd/csrc/sort.hip, compiled from
 // that source for the host, FA_REAL / FA_DIM on the command line) on the
-// coordinates in a binary file and writes the keys.
+// coordinates in a binary file and writes the keys; then, in 2-D and 3-D,
+// gathers the coordinates the key kernel packed into the stable key order
+// with the sort's gather kernel and writes them after the keys.
 //   bin_key_emu in.bin out.bin   (in: M, then geometry ints, then coordinates)
 #include "hip_shim.hpp"
 #include "bin_key_kernels.inc"
 
+#include <algorithm>
+#include <numeric>
 #include <vector>
 
 using namespace fa;
@@ -40,17 +44,39 @@ int main(int argc, char **argv) {
   }
   std::fclose(f);
   int *keys = static_cast<int *>(std::malloc(sizeof(int) * std::max(M, 1)));
+  // exactly the packed size (no packing in 1-D)
+  T *packed = DIM > 1 ? static_cast<T *>(std::malloc(sizeof(T) * pack_stride(DIM) *
+                                                     std::max(M, 1)))
+                      : nullptr;
   const T inv0 = T(1) / T(g.binsize[0]), inv1 = T(1) / T(g.binsize[1]),
           inv2 = T(1) / T(g.binsize[2]);
   const int nb = (M + 255) / 256;
   emu_launch(nb, 256, 0, [&] {
-    bin_key_kernel<T, DIM>(M, x[0], x[1], x[2], g, inv0, inv1, inv2, keys);
+    bin_key_kernel<T, DIM>(M, x[0], x[1], x[2], g, inv0, inv1, inv2, keys, packed);
   });
   FILE *o = std::fopen(argv[2], "wb");
   if (!o) return 2;
   std::fwrite(keys, sizeof(int), M, o);
+  if constexpr (DIM > 1) {
+    std::vector<int> idx(static_cast<size_t>(M));
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+    T *s[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < DIM; ++d) s[d] = static_cast<T *>(std::malloc(sizeof(T) * M));
+    emu_launch(nb, 256, 0, [&] {
+      gather_points_kernel<T, DIM>(idx.data(), packed, s[0], s[1], s[2], M);
+    });
+    for (int d = 0; d < DIM; ++d) {
+      for (int i = 0; i < M; ++i) {
+        const double v = double(s[d][i]);
+        std::fwrite(&v, sizeof(double), 1, o);
+      }
+      std::free(s[d]);
+    }
+  }
   std::fclose(o);
   for (int d = 0; d < DIM; ++d) std::free(x[d]);
   std::free(keys);
+  std::free(packed);
   return 0;
 }

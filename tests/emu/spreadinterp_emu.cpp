@@ -3,9 +3,11 @@
 // the command line) on seeded points and writes inputs and outputs for the
 // test to check against a direct FP64 sum. The point sort and subproblem
 // table are built here with the library's own first-node rule (start_index).
-//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|bin|binsub|mv|igm|grad>
+//   spreadinterp_emu <gm|sub|colored|col|colsub|colmix|od|bin|binsub|mv|igm|grad>
 //                    nf0 nf1 nf2 b0 b1 b2 M maxsub
 //                    blk clustered seed out.bin
+// (od: output-driven spreading with EMU_NP points staged per batch, default
+// 64.)
 // Exit codes: 0 ok, 2 usage, 3 geometry not valid for the variant, 4 the two
 // problems of the multi-vector launch differ.
 #include "hip_shim.hpp"
@@ -37,7 +39,7 @@ int main(int argc, char **argv) {
   const int clustered = std::atoi(argv[11]), seed = std::atoi(argv[12]);
   const char *out = argv[13];
   const bool col = var == "col" || var == "colsub" || var == "colmix";
-  const bool spreads = var == "gm" || var == "sub" || var == "colored" || col;
+  const bool spreads = var == "gm" || var == "sub" || var == "colored" || var == "od" || col;
   if (col && !(DIM == 3 && sizeof(T) == 4 && NS <= COL_MAX_NS && bs[0] == 8 && bs[1] == 8))
     return 3;
   if (var == "colored" || var == "col" || var == "colmix")
@@ -114,7 +116,7 @@ int main(int argc, char **argv) {
   SpreadCtx<T> ctx;
   ctx.dim = DIM;
   ctx.ns = NS;
-  ctx.method = var == "gm" || var == "igm" ? 1 : 2;
+  ctx.method = var == "gm" || var == "igm" ? 1 : var == "od" ? 3 : 2;
   ctx.M = M;
   for (int d = 0; d < 3; ++d) {
     ctx.nf[d] = nf[d];
@@ -139,6 +141,12 @@ int main(int argc, char **argv) {
   ctx.lds_bytes = size_t(ntile) * 2 * sizeof(T);
   ctx.lds_spread = size_t(ntile) * 2 * sizeof(double);
   ctx.interp_sub = var == "binsub" ? 1 : 0;
+  if (var == "od") {
+    const char *e = std::getenv("EMU_NP");
+    ctx.od_np = e ? std::atoi(e) : 64;
+    if (ctx.od_np < 1) return 2;
+    ctx.lds_od = ctx.lds_spread + od_stage_bytes(DIM, NS, ctx.od_np, sizeof(T));
+  }
   if (col) {
     ctx.colS = S;
     ctx.colstart = colstart;
@@ -159,23 +167,15 @@ int main(int argc, char **argv) {
     for (int64_t i = 0; i < int64_t(M) * blk; ++i) c[i] = {T(NAN), T(NAN)};
   }
 
-  // EMU_CSORTED=1: the LDS spreading kernels read the strengths in the
-  // sorted order (ctx.c_sorted), as the library's spreading plans do
-  C *cs = c;
-  if (spreads && var != "gm" && std::getenv("EMU_CSORTED") &&
-      std::string(std::getenv("EMU_CSORTED")) == "1") {
-    cs = alloc<C>(size_t(M) * blk);
-    for (int t = 0; t < blk; ++t)
-      for (int i = 0; i < M; ++i) cs[int64_t(t) * M + i] = c[int64_t(t) * M + idx[i]];
-    ctx.c_sorted = 1;
-  }
-
   if (var == "gm") {
     const int64_t g = (M + SPREAD_THREADS - 1) / SPREAD_THREADS;
     emu_launch(g, SPREAD_THREADS, 0, [&] { spread_gm_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
+  } else if (var == "od") {
+    emu_launch(cap, BIN_THREADS, ctx.lds_od,
+               [&] { spread_od_kernel<T, DIM, NS>(ctx, c, fw, nftot, ctx.od_np); }, blk);
   } else if (var == "sub") {
     emu_launch(cap, BIN_THREADS, ctx.lds_spread,
-               [&] { spread_sub_kernel<T, DIM, NS>(ctx, cs, fw, nftot); }, blk);
+               [&] { spread_sub_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
   } else if (var == "colored") {
     ctx.colored = 1;
     for (int color = 0; color < (1 << DIM); ++color) {
@@ -186,7 +186,7 @@ int main(int argc, char **argv) {
         n *= (nb[d] - col[d] + 1) / 2;
       }
       emu_launch(n, BIN_THREADS, ctx.lds_spread, [&] {
-        spread_bin_colored_kernel<T, DIM, NS>(ctx, cs, fw, nftot, col[0], col[1], col[2]);
+        spread_bin_colored_kernel<T, DIM, NS>(ctx, c, fw, nftot, col[0], col[1], col[2]);
       }, blk);
     }
   } else if (col) {
@@ -209,7 +209,7 @@ int main(int argc, char **argv) {
                      [&] { spread_col_sub_kernel<NS>(cx, cc, ff, nftot); }, blk);
       }
     };
-    launch_col(ctx, cs, fw);
+    launch_col(ctx, c, fw);
   } else if (var == "bin" || var == "binsub") {
     emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
                [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
@@ -275,7 +275,6 @@ int main(int argc, char **argv) {
   std::free(substart);
   std::free(sub2bin);
   std::free(colstart);
-  if (cs != c) std::free(cs);
   std::free(c);
   std::free(fw);
   return 0;

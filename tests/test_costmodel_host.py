@@ -69,6 +69,7 @@ def test_plan_args():
 def test_choice_is_the_predicted_fastest_allowed_option():
     key = costmodel.problem_class(2, 2, 1e-5, "complex64")
     m = _model(1, classes=[key, costmodel.problem_class(2, 2, 1e-6, "complex64")])
+    m.margin = 0.0
     prob = (2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", UNIFORM)
     y = m.predict(costmodel.features(*prob))[0]
     best = int(np.argmin(y))
@@ -84,6 +85,26 @@ def test_choice_is_the_predicted_fastest_allowed_option():
     assert m.choose(*prob, arms=allowed) in allowed
     eps, opts = m.plan_args(m.choose(*prob), 2, (256, 256), 1e-5)
     assert eps <= 1e-5
+
+
+def test_margin_keeps_the_defaults_for_small_predicted_gains():
+    """Another option replaces the defaults only when predicted faster by more
+    than the margin (log time)."""
+    key = costmodel.problem_class(2, 2, 1e-5, "complex64")
+    m = _model(2, classes=[key])
+    prob = (2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", UNIFORM)
+    y = m.predict(costmodel.features(*prob))[0]
+    allowed = [i for i in range(costmodel.NOUT)
+               if costmodel.arm_valid(costmodel.ARMS[i], 2, 2)]
+    best = min(allowed, key=lambda i: y[i])
+    gain = y[0] - y[best]
+    assert best != 0 and gain > 0  # this seed's network prefers another option
+    m.margin = gain + 1e-3
+    assert m.choose(*prob) == 0
+    m.margin = gain - 1e-3
+    assert m.choose(*prob) == best
+    assert costmodel.CostModel(m.params, m.mean, m.std, m.lo, m.hi,
+                               meta={"margin": 0.3}).margin == 0.3
 
 
 def test_unswept_class_or_default_miss_is_the_heuristic():

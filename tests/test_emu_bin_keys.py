@@ -2,9 +2,11 @@
 coordinate as include/cufinufft/spreadinterp.hpp:214-229, bins of the first
 support node, and their sub-column and LDS-bank refinements), compiled from
 its own source for the host with AddressSanitizer (tests/emu/hip_shim.hpp),
-bit-exact against oracle/sort.py. The keys fix the sort permutation; the
-radix passes that order them are checked bit-exact on the GPU
-(tests/test_gpu_parity.py::test_sort_permutation_bit_exact)."""
+bit-exact against oracle/sort.py; and, in 2-D and 3-D, the coordinates the
+key kernel packs per point, gathered into the stable key order by the
+sort's gather kernel, equal to the permuted inputs. The keys fix the sort
+permutation; the radix passes that order them are checked bit-exact on the
+GPU (tests/test_gpu_parity.py::test_sort_permutation_bit_exact)."""
 
 import shutil
 import subprocess
@@ -98,7 +100,15 @@ def test_bin_keys_vs_oracle(tmp_path_factory, tmp_path, case):
     res = subprocess.run([str(exe), str(inp), str(out)], capture_output=True, text=True,
                          timeout=300)
     assert res.returncode == 0, res.stderr[-3000:]
-    keys = np.fromfile(out, dtype=np.int32).astype(np.int64)
+    raw = np.fromfile(out, dtype=np.uint8)
+    keys = raw[:4 * M].view(np.int32).astype(np.int64)
+    if dim > 1:
+        # the coordinates packed by the key kernel, gathered in the stable
+        # key order by the sort's gather kernel: the permuted inputs, exactly
+        got = raw[4 * M:].view(np.float64).reshape(dim, M)
+        order = np.argsort(keys, kind="stable")
+        for d in range(dim):
+            assert np.array_equal(got[d], pts[d].astype(np.float64)[order])
     if mode == "ref":
         ref = osort.bin_keys(pts, list(nf), list(bs))
     else:

@@ -108,17 +108,16 @@ def _weights(xs, nf, ns, beta):
     return idx, w
 
 
-def _run(exe, tmp, var, nf, bs, M, maxsub, blk, clustered, seed, csorted=False):
-    """csorted: the spreading kernels read strengths already in the sorted
-    order (the library's LDS spreading plans scatter them there first)."""
+def _run(exe, tmp, var, nf, bs, M, maxsub, blk, clustered, seed, np_od=64):
+    """np_od: points staged per batch by the output-driven spreader (od)."""
     dim = len(nf)
     nf3 = list(nf) + [1] * (3 - dim)
     bs3 = list(bs) + [1] * (3 - dim)
     out = tmp / f"{var}.bin"
-    env = dict(os.environ, EMU_CSORTED="1" if csorted else "0")
     res = subprocess.run([str(exe), var, *map(str, nf3), *map(str, bs3), str(M), str(maxsub),
                           str(blk), str(clustered), str(seed), str(out)],
-                         capture_output=True, text=True, timeout=900, env=env)
+                         capture_output=True, text=True, timeout=900,
+                         env=dict(os.environ, EMU_NP=str(np_od)))
     assert res.returncode == 0, (res.returncode, res.stderr[-4000:])
     v = np.fromfile(out, dtype=np.float64)
     nsub = int(v[0])
@@ -145,16 +144,14 @@ GEOMS = [
 
 
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
-@pytest.mark.parametrize("var", ["gm", "sub", "colored", "sub-csorted", "colored-csorted"])
+@pytest.mark.parametrize("var", ["gm", "sub", "colored"])
 def test_spread_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
     real, dim, ns, nf, bs, M, maxsub, blk, clustered = geom
-    var, _, csorted = var.partition("-")
     if var == "colored" and not all(n % b == 0 and (n // b) % 2 == 0 and n // b >= 2
                                     and b >= ns - 1 for n, b in zip(nf, bs)):
         pytest.skip("bins not colorable (the library spreads such plans by subproblems)")
     exe = _exe(tmp_path_factory, real, dim, ns)
-    nsub, xs, c, fw = _run(exe, tmp_path, var, nf, bs, M, maxsub, blk, clustered, 11,
-                           csorted=bool(csorted))
+    nsub, xs, c, fw = _run(exe, tmp_path, var, nf, bs, M, maxsub, blk, clustered, 11)
     idx, w = _weights(xs, nf, ns, 2.30 * ns)
     ntot = int(np.prod(nf))
     for t in range(blk):
@@ -162,6 +159,26 @@ def test_spread_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
         np.add.at(ref, idx.ravel(), (w * c[t][:, None]).ravel())
         err = np.linalg.norm(fw[t] - ref) / np.linalg.norm(ref)
         print(var, geom, f"subproblems {nsub} relerr {err:.2e}")
+        assert err < (2e-6 if real == "float" else 1e-13)
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
+@pytest.mark.parametrize("np_od", [16, 64, 300])
+def test_output_driven_spread_vs_direct(tmp_path_factory, tmp_path, geom, np_od):
+    """Output-driven spreading (gpu_method 3): subproblems of maxsub points
+    staged np at a time (several batches per subproblem; np above the
+    workgroup's 256 threads too), against the direct spread."""
+    real, dim, ns, nf, bs, M, maxsub, blk, clustered = geom
+    exe = _exe(tmp_path_factory, real, dim, ns)
+    nsub, xs, c, fw = _run(exe, tmp_path, "od", nf, bs, M, maxsub, blk, clustered, 15,
+                           np_od=np_od)
+    idx, w = _weights(xs, nf, ns, 2.30 * ns)
+    ntot = int(np.prod(nf))
+    for t in range(blk):
+        ref = np.zeros(ntot, complex)
+        np.add.at(ref, idx.ravel(), (w * c[t][:, None]).ravel())
+        err = np.linalg.norm(fw[t] - ref) / np.linalg.norm(ref)
+        print("od", np_od, geom, f"subproblems {nsub} relerr {err:.2e}")
         assert err < (2e-6 if real == "float" else 1e-13)
 
 
@@ -189,19 +206,16 @@ COL_GEOMS = [
 
 
 @pytest.mark.parametrize("geom", COL_GEOMS, ids=lambda g: f"ns{g[0]}")
-@pytest.mark.parametrize("var,maxsub,clustered,csorted", [
-    ("col", 1024, 0, False), ("colsub", 100, 1, False), ("colmix", 8, 1, False),
-    ("col", 1024, 0, True), ("colmix", 8, 1, True)])
-def test_column_spreader_vs_direct(tmp_path_factory, tmp_path, geom, var, maxsub, clustered,
-                                   csorted):
+@pytest.mark.parametrize("var,maxsub,clustered", [("col", 1024, 0), ("colsub", 100, 1),
+                                                  ("colmix", 8, 1)])
+def test_column_spreader_vs_direct(tmp_path_factory, tmp_path, geom, var, maxsub, clustered):
     """Whole bins in parity colors (col), every bin as subproblems (colsub),
     and the mix the library runs on clustered points: bins above 8 maxsub
     points as subproblems, the rest colored (colmix)."""
     ns, nf, bs = geom
     exe = _exe(tmp_path_factory, "float", 3, ns)
     blk = 2 if var == "col" and ns == 4 else 1  # one multi-vector case (keeps the suite short)
-    nsub, xs, c, fw = _run(exe, tmp_path, var, nf, bs, 2000, maxsub, blk, clustered, 14,
-                           csorted=csorted)
+    nsub, xs, c, fw = _run(exe, tmp_path, var, nf, bs, 2000, maxsub, blk, clustered, 14)
     if var == "colmix":
         assert nsub > 0  # the cluster's bins went to the subproblem kernel
     idx, w = _weights(xs, nf, ns, 2.30 * ns)

@@ -4,12 +4,8 @@ sizes, clustered points cut into many subproblems, several vectors, and the
 adjoint of a method-3 plan. The reference runs the same matrix for method 3
 through test/cuda/CMakeLists.txt:26-250 (the method argument of its
 cufinufft*_test drivers) with the pass criteria of
-python/cufinufft/tests/utils.py:121-168.
-
-Skipped: the first GPU run of these tests, with a dedicated output-driven
-spreading kernel, ended with a GPU fault whose cause has not been found. The
-kernel is withheld (method 3 plans run the method-2 kernels, as the library
-documents), and the tests stay off the GPU until a cause is shown."""
+python/cufinufft/tests/utils.py:121-168. The kernel (spread_od_kernel) is
+also run from its source on the CPU (tests/test_emu_spreadinterp.py)."""
 
 import numpy as np
 import pytest
@@ -17,12 +13,7 @@ import pytest
 from oracle import nudft
 from tests.helpers import REAL, cnormal, rng, to_dev, to_host, uniform_pts
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the first GPU run of these tests (output-driven spreading kernel) "
-                            "ended with a GPU fault whose cause has not been found; the kernel "
-                            "is withheld and the tests stay off the GPU until a cause is shown"),
-]
+pytestmark = pytest.mark.gpu
 
 SHAPES = {1: (97,), 2: (24, 31), 3: (10, 13, 16)}
 TOL = {"complex64": 1e-5, "complex128": 1e-11}
@@ -119,7 +110,8 @@ def test_type3_vs_direct(torch_gpu, dim, dtype):
 @pytest.mark.parametrize("dtype,shape", [("complex64", (40, 48, 36)), ("complex128", (60, 50))])
 def test_adjointness(torch_gpu, dtype, shape):
     """<type1(c), f> = <c, type1^H(f)> on a method-3 plan: its adjoint
-    interpolates with the bins chosen for output-driven spreading."""
+    interpolates (LDS method) with the bins chosen for output-driven
+    spreading."""
     from finufft_amd import Plan
 
     torch = torch_gpu
@@ -141,9 +133,8 @@ def test_adjointness(torch_gpu, dtype, shape):
 
 def test_matches_method2(torch_gpu):
     """Methods 2 and 3 spread the same points to the same grid (spread-only
-    plans, so the comparison is of the spreading alone), up to the fp32
-    rounding of sums over about a thousand points per cell (the column
-    spreader of method 2 sums in fp64, method 3 in the plan precision)."""
+    plans, so the comparison is of the spreading alone); both sum in fp64
+    tiles, in different orders."""
     from finufft_amd import Plan
 
     torch = torch_gpu
