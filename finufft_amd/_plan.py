@@ -1,10 +1,14 @@
 """Python ``Plan`` over the C ABI, with the interface of the reference's
 ``cufinufft.Plan`` (python/cufinufft/cufinufft/_plan.py:39-400).
 
-Arrays are torch tensors on the GPU. As in the reference wrapper, ``n_modes``
-is given in array-shape order (slowest axis first) and the points as
-``(x, y, z)`` along those axes; both are reversed on the way into the
-x-fastest C layout.
+Arrays are torch tensors on the GPU, or any GPU array that exposes
+``__cuda_array_interface__`` (cupy, numba, pycuda) or DLPack
+(``__dlpack__``), taken without a copy, as the reference's wrapper takes
+cupy, numba, pycuda and torch arrays (python/cufinufft/cufinufft/_compat.py:
+6-111). Results come back as torch tensors, or in the caller's ``out``
+array. As in the reference wrapper, ``n_modes`` is given in array-shape
+order (slowest axis first) and the points as ``(x, y, z)`` along those
+axes; both are reversed on the way into the x-fastest C layout.
 """
 
 from __future__ import annotations
@@ -42,6 +46,17 @@ def _dtype_name(dtype) -> str:
 
 def _ptr(t) -> int | None:
     return None if t is None else t.data_ptr()
+
+
+def _as_tensor(t):
+    """A torch view of a GPU array of another framework (no copy), or t."""
+    if torch is None or isinstance(t, torch.Tensor):
+        return t
+    if hasattr(t, "__cuda_array_interface__"):
+        return torch.as_tensor(t, device="cuda")
+    if hasattr(t, "__dlpack__"):
+        return torch.from_dlpack(t)
+    return t
 
 
 class Plan:
@@ -169,8 +184,10 @@ class Plan:
     def _check(self, t, name, real, output=False):
         if t is None:
             return None
+        t = _as_tensor(t)
         if torch is None or not isinstance(t, torch.Tensor):
-            raise TypeError(f"Argument `{name}` must be a torch tensor on the GPU")
+            raise TypeError(f"Argument `{name}` must be a GPU array (a torch tensor, or an "
+                            "array with __cuda_array_interface__ or __dlpack__)")
         want = getattr(torch, self._rname if real else self._cname)
         if t.dtype != want:
             raise TypeError(f"Argument `{name}` does not have the correct dtype: "
@@ -244,6 +261,7 @@ class Plan:
         else:
             raise ValueError(f"Invalid data shape {tuple(data.shape)}, expected "
                              f"{(n, *in_shape) if n > 1 else in_shape}")
+        given = out
         if out is None:
             out = torch.empty(batch + out_shape, dtype=data.dtype, device=data.device)
             self._hold_for_plan_stream(out)
@@ -261,7 +279,8 @@ class Plan:
         ier = getattr(_lib.lib(), fn)(self._plan, c_ptr, fk_ptr)
         if ier != 0:
             raise FinufftError(fn, ier)
-        return out
+        # the caller's own output array (of any framework) when one was given
+        return out if given is None or isinstance(given, torch.Tensor) else given
 
     def execute(self, data, out=None):
         """Type 1: strengths -> modes; type 2: modes -> values; type 3: strengths -> targets."""

@@ -13,8 +13,8 @@ for the options (ARMS): the library's defaults (LDS methods, upsampling 2,
 the kernel width for tol), the global-memory method (gpu_method 1), all
 vectors of an execute in one pass (gpu_maxbatchsize 32), upsampling factor
 1.25, a kernel one width class wider (the plan made for tol / 10), two
-alternative bin shapes per (dimension, type) (BIN_SHAPES), and the
-global-memory method with one pass over the vectors.
+alternative bin shapes per (dimension, type) (BIN_SHAPES), and output-driven
+spreading (gpu_method 3; types 1 and 3 only).
 
 Training data come from sweeps of the library's own options on the GPU
 (`sweep`): every option is timed and its error against the FP64 direct sum
@@ -58,7 +58,7 @@ ARMS = [
     {"eps_scale": 0.1},
     {"bins": 0},
     {"bins": 1},
-    {"gpu_method": 1, "gpu_maxbatchsize": 32},
+    {"gpu_method": 3},
 ]
 # Alternatives to the heuristic's bin shape, (x, y[, z]) with x fastest, per
 # (dimension, type): shallower column bins and wide LDS tiles for spreading,
@@ -101,6 +101,10 @@ def problem_class(ttype, dim, tol, dtype):
 
 
 def arm_valid(arm, ttype, dim):
+    """Whether option `arm` applies: bin shapes where BIN_SHAPES has them,
+    output-driven spreading for the types that spread."""
+    if arm.get("gpu_method") == 3 and ttype == 2:
+        return False
     return "bins" not in arm or (dim, ttype) in BIN_SHAPES
 
 
@@ -374,8 +378,16 @@ def time_arm(ttype, n_modes, pts, ntransf, tol, dtype, arm, reps=5, seed=0, err_
     if ntransf == 1:
         data = data[0]
     out = plan.execute(data)
+    plan.execute(data, out=out)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # at least `reps` executes and about 3 ms of them: single small problems
+    # run for tens of microseconds, where a few repetitions time mostly noise
+    t0 = time.perf_counter()
+    plan.execute(data, out=out)
+    torch.cuda.synchronize()
+    one = max(time.perf_counter() - t0, 1e-6)
+    reps = int(min(200, max(reps, math.ceil(3e-3 / one))))
     ev0.record()
     for _ in range(reps):
         plan.execute(data, out=out)
@@ -402,8 +414,9 @@ def sweep_configs(seed=0, n=200):
     """Problem configurations of the sweep: 2-D and 3-D, types 1 and 2,
     uniform, clustered and radial-spoke points, fp32 (three in four) and
     fp64, sizes around the uneven small-problem, the 2-D and the 3-D
-    workloads; plus, for the first sweep, the 3-D headline shape (256^3
-    modes, 1e8 points, uniform and spokes) and the 2-D one (1000^2, 1e7)."""
+    workloads; problems drawn as the uneven small-problem workload's; plus,
+    for the first sweep, the 3-D headline shape (256^3 modes, 1e8 points,
+    uniform and spokes) and the 2-D one (1000^2, 1e7)."""
     r = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -422,6 +435,11 @@ def sweep_configs(seed=0, n=200):
         else:
             dtype, tol = "complex128", float(r.choice([1e-6, 1e-9, 1e-12]))
         out.append((ttype, (side,) * dim, M, ntr, tol, dtype, kind))
+    # problems of the uneven small-problem workload (2-D type 2, 32 vectors)
+    for _ in range(24 if seed == 0 else 8):
+        side = int(np.exp(r.uniform(np.log(32), np.log(512))))
+        M = int(np.exp(r.uniform(np.log(1e3), np.log(1e5))))
+        out.append((2, (side, side), M, 32, 1e-5, "complex64", "uniform"))
     if seed == 0:
         for ttype in (1, 2):
             for kind in ("uniform", "spokes"):

@@ -592,7 +592,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 // A workgroup's points (a bin, or a subproblem of a dense bin) are dealt to
 // its waves as equal contiguous ranges of the sorted order, in which the
 // points of a sub-column are contiguous; a sub-column split between two waves
-// is added to the tile by both. A wave stages 64 points at a time: lane k
+// is added to the tile by both. A wave stages COL_BATCH points at a time: lane k
 // evaluates the kernel of point p + k and writes a record of COL_REC floats:
 // wz (8 floats, zero past ns: two broadcast 128-bit reads), then c wx as
 // (re, im) pairs shifted to the lanes' columns (16 floats, one 64-bit read
@@ -697,9 +697,11 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
   // The first batch's loads are issued before the workgroup's start-up work
   // (clearing the tile), and each next batch's strengths while the current
   // batch is walked, so that memory latency overlaps work.
-  fetch(u0 + lane);
+  // lanes below COL_BATCH hold one point of the batch each
+  const bool stager = lane < COL_BATCH;
+  if (stager) fetch(u0 + lane);
   float2 fc = make_float2(0.f, 0.f);
-  if (u0 + lane < u1) fc = c[fj];
+  if (stager && u0 + lane < u1) fc = c[fj];
   start();
 
   float ar[ZD], ai[ZD];
@@ -709,11 +711,11 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
   int zlo = ZD, zhi = 0;  // planes [zlo, zhi) touched since the last flush
   int colcell = 0;
 
-  for (int p = u0; p < u1; p += 64) {
-    const int n = min(64, u1 - p);
+  for (int p = u0; p < u1; p += COL_BATCH) {
+    const int n = min(COL_BATCH, u1 - p);
     const float x = fx, y = fy, z = fz;
     const float2 cv = fc;
-    fetch(p + 64 + lane);
+    if (stager) fetch(p + COL_BATCH + lane);
     int posv = 0;  // lz + 64 * sub-column of this lane's point
     if (lane < n) {
       float kx[NS], ky[NS], kz[NS];
@@ -757,7 +759,7 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
       r[6] = make_float4(wy8[0], wy8[1], wy8[2], wy8[3]);
       r[7] = make_float4(wy8[4], wy8[5], wy8[6], wy8[7]);
     }
-    if (p + 64 + lane < u1) fc = c[fj];  // the next batch's strengths
+    if (stager && p + COL_BATCH + lane < u1) fc = c[fj];  // the next batch's strengths
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -795,7 +797,7 @@ __device__ __forceinline__ void col_spread_unit(const SpreadCtx<float> &ctx,
   const int u1 = first + int(int64_t(npts) * (wave + 1) / COL_WAVES);
   // every wave runs the range (an empty one too): the tile is cleared and
   // the workgroup synchronized inside, after the first loads are issued
-  col_spread_range<NS>(ctx, st, c, re, im, stage_all + wave * (64 * COL_REC), u0, u1, [&] {
+  col_spread_range<NS>(ctx, st, c, re, im, stage_all + wave * (COL_BATCH * COL_REC), u0, u1, [&] {
     for (int n = threadIdx.x; n < st.ntile; n += 64 * COL_WAVES) {
       re[n] = 0.0;
       im[n] = 0.0;
@@ -998,7 +1000,9 @@ __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const Start
   // Strided dealing: work item i = 32 g + m takes the unit's point m G + g
   // (G = ceil(npts / 32)). When the bin is ordered by LDS bank class (the
   // sort's bank refinement), the 32 lanes of a half-wave then start their
-  // gathers in 32 different bank classes.
+  // gathers in 32 different bank classes, in class order. (Dealing classes
+  // 4 apart to each 8 consecutive lanes instead doubled the kernel's bank
+  // conflict cycles.)
   const int G = (st.npts + 31) >> 5;
   for (int i = threadIdx.x; i < 32 * G; i += BIN_THREADS) {
     const int pos = (i & 31) * G + (i >> 5);

@@ -112,15 +112,18 @@ __device__ __forceinline__ kptr<P> to_const(const P *p) {
 // 8 x 8 (x, y) grid columns, one per lane, for the points whose first node
 // lies in an S x S square, S the largest power of two with S + ns - 1 <= 8;
 // each lane holds its column's whole depth in the bin's tile (bin depth at
-// most COL_ZC) in registers. The tile is fp64; every wave stages 64 points of
-// COL_REC floats.
+// most COL_ZC) in registers. The tile is fp64; every wave stages COL_BATCH
+// points of COL_REC floats at a time. Eight waves with 32-point batches keep
+// the workgroup's LDS at two workgroups per CU (16 waves) where four waves
+// with 64-point batches gave 8 waves per CU.
 constexpr int COL_MAX_NS = 8;
-constexpr int COL_WAVES = 4;  // waves per workgroup
+constexpr int COL_WAVES = 8;  // waves per workgroup
+constexpr int COL_BATCH = 32; // points a wave stages at a time (<= 64)
 constexpr int COL_REC = 32;   // floats staged per point
 constexpr int COL_ZC = 8;     // largest bin depth (first-node planes)
 inline constexpr int col_side(int ns) { return ns <= 5 ? 4 : (ns <= 7 ? 2 : 1); }
 inline constexpr size_t col_stage_bytes() {
-  return size_t(COL_WAVES) * 64 * COL_REC * sizeof(float);
+  return size_t(COL_WAVES) * COL_BATCH * COL_REC * sizeof(float);
 }
 // LDS bytes of a column-spreading unit with a tile of `cells` cells.
 inline constexpr size_t col_lds_bytes(int64_t cells) {

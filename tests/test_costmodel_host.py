@@ -64,6 +64,9 @@ def test_plan_args():
     assert costmodel.plan_args({"gpu_method": 1, "gpu_maxbatchsize": 32}, 2, (8, 8), 1e-3) == (
         1e-3, {"gpu_method": 1, "gpu_maxbatchsize": 32})
     assert not costmodel.arm_valid({"bins": 0}, 3, 3) and costmodel.arm_valid({}, 3, 1)
+    # output-driven spreading only for the types that spread
+    assert not costmodel.arm_valid({"gpu_method": 3}, 2, 3)
+    assert costmodel.arm_valid({"gpu_method": 3}, 1, 3) and costmodel.arm_valid({"gpu_method": 3}, 3, 2)
 
 
 def test_choice_is_the_predicted_fastest_allowed_option():
@@ -71,15 +74,19 @@ def test_choice_is_the_predicted_fastest_allowed_option():
     m = _model(1, classes=[key, costmodel.problem_class(2, 2, 1e-6, "complex64")])
     m.margin = 0.0
     prob = (2, (256, 256), 10 ** 5, 32, 1e-5, "complex64", UNIFORM)
+    # the options that apply to 2-D type 2, in order of predicted time
+    valid = [i for i in range(costmodel.NOUT) if costmodel.arm_valid(costmodel.ARMS[i], 2, 2)]
     y = m.predict(costmodel.features(*prob))[0]
-    best = int(np.argmin(y))
+    ranked = [int(i) for i in np.argsort(y) if int(i) in valid]
+    best = ranked[0]
     assert m.choose(*prob) == best
     # a rejected option is skipped for its class only
     m.rejected[key] = [best]
-    second = [int(i) for i in np.argsort(y) if int(i) != best][0]
+    second = ranked[1]
     assert m.choose(*prob) == second
     other = (2, (256, 256), 10 ** 5, 32, 1e-6, "complex64", UNIFORM)
-    assert m.choose(*other) == int(np.argmin(m.predict(costmodel.features(*other))[0]))
+    y2 = m.predict(costmodel.features(*other))[0]
+    assert m.choose(*other) == min(valid, key=lambda i: y2[i])
     # the candidates filter
     allowed = [i for i in range(costmodel.NOUT) if i not in (best, second)]
     assert m.choose(*prob, arms=allowed) in allowed
