@@ -99,6 +99,8 @@ def _declare(lib):
     lib.finufft_amd_set_stage_timing.restype = c_int
     lib.finufft_amd_get_stage_times.argtypes = [vp, c_int, POINTER(c_double)]
     lib.finufft_amd_get_stage_times.restype = c_int
+    lib.finufft_amd_set_fft_path.argtypes = [c_int]
+    lib.finufft_amd_set_fft_path.restype = c_int
     lib.finufft_amd_density_stats.argtypes = [c_int, c_int64, vp, vp, vp, c_int, c_int,
                                               POINTER(c_double), vp]
     lib.finufft_amd_density_stats.restype = c_int

@@ -163,6 +163,7 @@ class Plan:
             "binsize": tuple(v[7:10]), "batchsize": v[10], "M": v[11], "nbins": v[12],
             "col_side": v[13], "colored": v[14], "dense_subproblems": v[15],
             "beta": dout[0], "upsampfac": dout[1], "np": int(dout[2]),
+            "fft_path": int(dout[3]),
         }
 
     STAGES = ("zero", "spread", "fft", "deconvolve", "interp", "type3")

@@ -45,7 +45,7 @@ def _units():
     units = []
     for src in ["host_math.cpp", "fft.cpp", "c_api.cpp", "batch.cpp", "plan.hip",
                 "plan_kernels.hip", "sort.hip", "mlp.hip", "fused_fft.hip", "probe.hip",
-                "density.hip"]:
+                "density.hip", "fft3.hip"]:
         units.append((CSRC / src, BUILD / (Path(src).stem + ".o"), []))
     for real in ("float", "double"):
         for dim in (1, 2, 3):

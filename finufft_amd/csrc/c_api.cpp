@@ -360,7 +360,7 @@ int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout, double
         dout[0] = p->beta();
         dout[1] = p->upsampfac();
         dout[2] = p->od_np();
-        dout[3] = 0;
+        dout[3] = p->fft_path();
       }
     };
     void *q = const_cast<void *>(plan);
@@ -400,6 +400,13 @@ int finufft_amd_set_stage_timing(void *plan, int is_double, int on) {
       as_plan<double>(plan)->timer().enable(on != 0);
     else
       as_plan<float>(plan)->timer().enable(on != 0);
+  });
+}
+
+int finufft_amd_set_fft_path(int path) {
+  return safe_call([&] {
+    if (path != 0 && path != 1) fa::fail(FINUFFT_ERR_INVALID_ARGUMENT, "fft path must be 0 or 1");
+    fa::set_fft_path_setting(path);
   });
 }
 

@@ -1,6 +1,8 @@
 // NUFFT plan implementation (see plan.hpp).
 #include "plan.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -512,6 +514,13 @@ void Plan<T>::setup_bins() {
   }
 }
 
+namespace {
+std::atomic<int> g_fft_path{0};
+}  // namespace
+
+int fft_path_setting() { return g_fft_path.load(std::memory_order_relaxed); }
+void set_fft_path_setting(int path) { g_fft_path.store(path, std::memory_order_relaxed); }
+
 template <typename T>
 void Plan<T>::prepare_grid() {
   // The work grid, the FFT plan and the kernel's Fourier series.
@@ -519,7 +528,13 @@ void Plan<T>::prepare_grid() {
     fw_.resize(size_t(nftot_) * batchsize_);
   }
   if (!spread_only_) {
-    fft_.setup(dim_, nf_, batchsize_, std::is_same<T, double>::value);
+    fft3_.reset();
+    if (fft_path_setting() == 0 && fft3_supported(dim_, nf_)) {
+      fft3_ = std::make_unique<Fft3<T>>();
+      fft3_->setup(ms_, nf_, batchsize_, opts_.modeord);
+    } else {
+      fft_.setup(dim_, nf_, batchsize_, std::is_same<T, double>::value);
+    }
     int q = 0;
     std::vector<double> z(3 * MAX_NQUAD, 0.0), f(3 * MAX_NQUAD, 0.0);
     const KernelFn ker(formula_, ns_, beta_);
@@ -749,7 +764,6 @@ void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
     return;
   }
   const SpreadCtx<T> ctx = spread_ctx();
-  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
   for (int i = 0; i * batchsize_ < ntr; ++i) {
     const int blk = std::min(ntr - i * batchsize_, batchsize_);
     const C *c = d_c + int64_t(i) * batchsize_ * M_;
@@ -762,20 +776,48 @@ void Plan<T>::exec_type1(C *d_c, C *d_fk, int fft_sign, int ntr) {
     spread<T>(ctx, c, fw, blk, stream_);
     timer_.end(stream_);
     if (spread_only_) continue;
-    timer_.begin(ST_FFT, stream_);
-    fft_.exec(fw, fft_sign, stream_);
-    timer_.end(stream_);
-    timer_.begin(ST_DECONV, stream_);
-    deconvolve<T>(dim_, opts_.modeord, true, ms_, nf_, ker, fw, fk, blk, stream_);
-    timer_.end(stream_);
+    grid_to_modes(fw, fk, fft_sign, blk);
   }
+}
+
+template <typename T>
+void Plan<T>::grid_to_modes(C *fw, C *fk, int fft_sign, int blk) {
+  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
+  if (fft3_) {
+    timer_.begin(ST_FFT, stream_);
+    fft3_->to_modes(fw, fk, ker, fft_sign, blk, stream_);
+    timer_.end(stream_);
+    return;
+  }
+  timer_.begin(ST_FFT, stream_);
+  fft_.exec(fw, fft_sign, stream_);
+  timer_.end(stream_);
+  timer_.begin(ST_DECONV, stream_);
+  deconvolve<T>(dim_, opts_.modeord, true, ms_, nf_, ker, fw, fk, blk, stream_);
+  timer_.end(stream_);
+}
+
+template <typename T>
+void Plan<T>::modes_to_grid(C *fk, C *fw, int fft_sign, int blk) {
+  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
+  if (fft3_) {
+    timer_.begin(ST_FFT, stream_);
+    fft3_->to_grid(fk, fw, ker, fft_sign, blk, stream_);
+    timer_.end(stream_);
+    return;
+  }
+  timer_.begin(ST_DECONV, stream_);
+  deconvolve<T>(dim_, opts_.modeord, false, ms_, nf_, ker, fw, fk, blk, stream_);
+  timer_.end(stream_);
+  timer_.begin(ST_FFT, stream_);
+  fft_.exec(fw, fft_sign, stream_);
+  timer_.end(stream_);
 }
 
 template <typename T>
 void Plan<T>::exec_type2(C *d_c, C *d_fk, int fft_sign, int ntr) {
   if (M_ == 0) return;  // no points: no values to compute
   const SpreadCtx<T> ctx = spread_ctx();
-  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
   for (int i = 0; i * batchsize_ < ntr; ++i) {
     const int blk = std::min(ntr - i * batchsize_, batchsize_);
     C *c = d_c + int64_t(i) * batchsize_ * M_;
@@ -783,12 +825,7 @@ void Plan<T>::exec_type2(C *d_c, C *d_fk, int fft_sign, int ntr) {
     C *fw = fk;
     if (!spread_only_) {
       fw = fw_.get();
-      timer_.begin(ST_DECONV, stream_);
-      deconvolve<T>(dim_, opts_.modeord, false, ms_, nf_, ker, fw, fk, blk, stream_);
-      timer_.end(stream_);
-      timer_.begin(ST_FFT, stream_);
-      fft_.exec(fw, fft_sign, stream_);
-      timer_.end(stream_);
+      modes_to_grid(fk, fw, fft_sign, blk);
     }
     timer_.begin(ST_INTERP, stream_);
     interp<T>(ctx, c, fw, blk, stream_);
@@ -857,15 +894,13 @@ void Plan<T>::interp_grad(C *d_c, C *d_dc, C *d_fk) {
   if (M_ == 0) return;  // no points: no values or derivatives
   const int fft_sign = type_ == 2 ? iflag_ : -iflag_;
   const SpreadCtx<T> ctx = spread_ctx();
-  const T *ker[3] = {fwker_[0].get(), fwker_[1].get(), fwker_[2].get()};
   const int64_t mtot = int64_t(M_) * ntransf_;
   for (int i = 0; i * batchsize_ < ntransf_; ++i) {
     const int blk = std::min(ntransf_ - i * batchsize_, batchsize_);
     C *c = d_c ? d_c + int64_t(i) * batchsize_ * M_ : nullptr;
     C *fk = d_fk + int64_t(i) * batchsize_ * nmodes_;
     C *fw = fw_.get();
-    deconvolve<T>(dim_, opts_.modeord, false, ms_, nf_, ker, fw, fk, blk, stream_);
-    fft_.exec(fw, fft_sign, stream_);
+    modes_to_grid(fk, fw, fft_sign, blk);
     // dc is (dim, ntransf, M); this batch writes vectors [i b, i b + blk) of
     // every dimension, so the kernel's per-dimension stride must be mtot.
     if (blk == ntransf_) {

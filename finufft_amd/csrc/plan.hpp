@@ -21,11 +21,18 @@
 
 #include "common.hpp"
 #include "fft.hpp"
+#include "fft3.hpp"
 #include "host_math.hpp"
 #include "sort.hpp"
 #include "spreadinterp.hpp"
 
 namespace fa {
+
+// Grid <-> modes transforms of plans made from now on: 0 (the default) the
+// pruned line passes where the grid allows them, 1 rocFFT with a separate
+// amplification / deconvolution pass.
+int fft_path_setting();
+void set_fft_path_setting(int path);
 
 template <typename T>
 class BatchPlan;
@@ -71,6 +78,9 @@ class Plan {
   int col_side() const { return geom_.colS; }
   int colored() const { return colored_ ? 1 : 0; }
   int dense_subproblems() const { return n_dense_sub_; }
+  // 1 when the grid <-> modes transforms run as pruned line passes (3-D
+  // power-of-two grids), 0 when rocFFT with separate (de)convolution.
+  int fft_path() const { return fft3_ ? 1 : 0; }
   hipStream_t stream() const { return stream_; }
   StageTimer &timer() { return timer_; }
   // Number of setpts calls so far (a batch records it and refuses to run
@@ -91,6 +101,10 @@ class Plan {
   void exec_type2(C *c, C *fk, int fft_sign, int ntr);
   void exec_type3(C *c, C *fk);
   void prepare_grid();
+  // fw (blk vectors) -> fk with the deconvolution (type-1 direction; fw is
+  // overwritten), and fk -> fw with the amplification (type-2 direction).
+  void grid_to_modes(C *fw, C *fk, int fft_sign, int blk);
+  void modes_to_grid(C *fk, C *fw, int fft_sign, int blk);
 
   // parameters
   int type_, dim_, ntransf_, iflag_;
@@ -140,6 +154,7 @@ class Plan {
   // fine grid work buffer and FFT
   dbuf<C> fw_;
   FftPlan fft_;
+  std::unique_ptr<Fft3<T>> fft3_;  // pruned line transforms, when supported
   dbuf<C> grad_scratch_;  // interp_grad when ntransf exceeds the batch size
 
   // type 3

@@ -119,7 +119,17 @@ FINUFFT_EXPORT int finufft_amd_density_stats_noalloc(int dim, int64_t M, const v
  *           spreader (0 when unused), whole-bin parity-colored spreading (0/1),
  *           number of dense-bin subproblems.
  * dout[4]:  beta, upsampfac, points staged per batch by output-driven
- *           spreading (gpu_method 3; else 0), 0. */
+ *           spreading (gpu_method 3; else 0), 1 when the grid <-> modes
+ *           transforms run as pruned line passes (else rocFFT, 0). */
+/* Grid <-> modes transforms of the plans made after the call (no reference
+ * counterpart; the reference always runs a full cuFFT transform plus a
+ * separate deconvolution / amplification pass, src/cuda/execute.cu:105-204).
+ * path 0 (the default): on 3-D grids whose sides are powers of two in
+ * [16, 4096], three pruned line-transform passes with the deconvolution or
+ * amplification folded in; rocFFT otherwise. path 1: rocFFT and the separate
+ * pass for every plan. Process-wide; other values -> 21 (invalid argument). */
+FINUFFT_EXPORT int finufft_amd_set_fft_path(int path);
+
 FINUFFT_EXPORT int finufft_amd_plan_info(const void *plan, int is_double, int64_t *iout,
                                          double *dout);
 

@@ -91,6 +91,10 @@ def test_error_codes_before_device():
     lib = _lib.lib()
     assert lib.cufinufft_destroy(None) == 16  # FINUFFT_ERR_PLAN_NOTVALID
     assert lib.cufinufftf_destroy(None) == 16
+    # the process-wide transform path: 0 or 1, anything else refused
+    assert lib.finufft_amd_set_fft_path(2) == 21
+    assert lib.finufft_amd_set_fft_path(-1) == 21
+    assert lib.finufft_amd_set_fft_path(0) == 0
     assert lib.cufinufftf_execute(None, None, None) == 16
     assert lib.cufinufftf_setpts(None, 1, None, None, None, 0, None, None, None) == 16
     # a pointer that is not a live plan (never made, destroyed, or of the other
