@@ -107,6 +107,29 @@ struct KerParams {
   const T *coef;  // device: coef[k*ns + i], k = 0..NC(ns)-1, ascending powers
 };
 
+// Intervals I and NS-1-I of the Horner table at z (w = z^2) from the even
+// and odd parts of interval I's polynomial, then the pairs further out, down
+// to I = 0 (compile-time indices throughout, so that ker stays in
+// registers; the last value written is ker[NS-1], as in the direct
+// evaluation, so that the two branches of eval_kernel_c end alike and no
+// store is merged across them into a run-time index).
+template <typename T, int NS, int I, typename CP>
+__device__ __forceinline__ void horner_pairs(T *ker, T z, T w, CP coef) {
+  if constexpr (I >= 0) {
+    constexpr int NC = horner_nc_sigma2(NS);
+    constexpr int NE = (NC + 1) / 2, NO = NC / 2;  // even / odd coefficients
+    T e = coef[2 * (NE - 1) * NS + I];
+#pragma unroll
+    for (int j = NE - 2; j >= 0; --j) e = std::fma(e, w, coef[2 * j * NS + I]);
+    T o = coef[(2 * NO - 1) * NS + I];
+#pragma unroll
+    for (int j = NO - 2; j >= 0; --j) o = std::fma(o, w, coef[(2 * j + 1) * NS + I]);
+    ker[I] = std::fma(z, o, e);
+    if constexpr (NS - 1 - I != I) ker[NS - 1 - I] = std::fma(-z, o, e);
+    horner_pairs<T, NS, I - 1>(ker, z, w, coef);
+  }
+}
+
 // Fills ker[i] = phi at the ns grid offsets for a point whose first grid node
 // sits at xl = s - x_rescaled in [-ns/2, -ns/2 + 1]; z = 2 xl + ns - 1 is the
 // same offset scaled to [-1, 1] (the Horner variable).
@@ -116,16 +139,14 @@ template <typename T, int NS, typename CP>
 __device__ __forceinline__ void eval_kernel_c(T *ker, T xl, T z, const KerParams<T> &kp,
                                               CP coef) {
   if (kp.horner) {
-    constexpr int NC = horner_nc_sigma2(NS);
-#pragma unroll
-    for (int i = 0; i < NS; ++i) ker[i] = coef[(NC - 1) * NS + i];
-#pragma unroll
-    for (int k = NC - 2; k >= 0; --k) {
-#pragma unroll
-      for (int i = 0; i < NS; ++i) ker[i] = std::fma(ker[i], z, coef[k * NS + i]);
-    }
+    // The kernel is even, so interval NS-1-i's polynomial is interval i's
+    // reflected: p_{NS-1-i}(z) = p_i(-z) (the tables are symmetric to
+    // rounding). With p_i(z) = E_i(z^2) + z O_i(z^2), one evaluation of the
+    // even and odd parts gives both values of a pair: about NC FMAs per pair
+    // instead of 2 (NC - 1).
+    horner_pairs<T, NS, (NS - 1) / 2>(ker, z, z * z, coef);
   } else {
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int i = 0; i < NS; ++i) {
       const T a = xl + T(i);
       const T zsq = kp.es_c * a * a;

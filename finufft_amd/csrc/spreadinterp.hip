@@ -618,11 +618,11 @@ struct ColRec {
 // (lz + 64 * sub-column) whose first plane is Z, then the runs of planes
 // Z + 1 .. COL_ZC - 1.
 template <int NS, int ZD, int Z>
-__device__ __forceinline__ void col_walk(int &k, int n, int base, int posv, const float *stage,
-                                         int xoff, int yoff, float (&ar)[ZD], float (&ai)[ZD],
-                                         int &zlo, int &zhi) {
+__device__ __forceinline__ void col_walk(int &k, int n, int kb, int base, int posv,
+                                         const float *stage, int xoff, int yoff, float (&ar)[ZD],
+                                         float (&ai)[ZD], int &zlo, int &zhi) {
   if constexpr (Z < COL_ZC) {
-    if (k < n && __builtin_amdgcn_readlane(posv, k) == base + Z) {
+    if (k < n && __builtin_amdgcn_readlane(posv, kb + k) == base + Z) {
       zlo = min(zlo, Z);
       zhi = max(zhi, Z + NS);
       do {
@@ -637,9 +637,9 @@ __device__ __forceinline__ void col_walk(int &k, int n, int base, int posv, cons
           ai[Z + m] = std::fma(wi, wz[m], ai[Z + m]);
         }
         ++k;
-      } while (k < n && __builtin_amdgcn_readlane(posv, k) == base + Z);
+      } while (k < n && __builtin_amdgcn_readlane(posv, kb + k) == base + Z);
     }
-    col_walk<NS, ZD, Z + 1>(k, n, base, posv, stage, xoff, yoff, ar, ai, zlo, zhi);
+    col_walk<NS, ZD, Z + 1>(k, n, kb, base, posv, stage, xoff, yoff, ar, ai, zlo, zhi);
   }
 }
 
@@ -694,14 +694,16 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
       fz = coord(ctx, 2, q, fj);
     }
   };
-  // The first batch's loads are issued before the workgroup's start-up work
-  // (clearing the tile), and each next batch's strengths while the current
-  // batch is walked, so that memory latency overlaps work.
-  // lanes below COL_BATCH hold one point of the batch each
-  const bool stager = lane < COL_BATCH;
-  if (stager) fetch(u0 + lane);
+  // Every lane evaluates one point of a pass of 64 and keeps its record in
+  // registers; the records go to the wave's stage (COL_BATCH = 32 records)
+  // one half-pass at a time, each half walked before the next is written.
+  // The first pass's loads are issued before the workgroup's start-up work
+  // (clearing the tile), and each next pass's loads while the current pass
+  // is walked, so that memory latency overlaps work.
+  static_assert(COL_BATCH == 32, "two half-passes of 32 records per pass of 64 points");
+  fetch(u0 + lane);
   float2 fc = make_float2(0.f, 0.f);
-  if (stager && u0 + lane < u1) fc = c[fj];
+  if (u0 + lane < u1) fc = c[fj];
   start();
 
   float ar[ZD], ai[ZD];
@@ -711,12 +713,13 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
   int zlo = ZD, zhi = 0;  // planes [zlo, zhi) touched since the last flush
   int colcell = 0;
 
-  for (int p = u0; p < u1; p += COL_BATCH) {
-    const int n = min(COL_BATCH, u1 - p);
+  for (int p = u0; p < u1; p += 64) {
+    const int n = min(64, u1 - p);
     const float x = fx, y = fy, z = fz;
     const float2 cv = fc;
-    if (stager) fetch(p + COL_BATCH + lane);
+    fetch(p + 64 + lane);
     int posv = 0;  // lz + 64 * sub-column of this lane's point
+    float4 rec[8];  // this lane's point's record (COL_REC floats)
     if (lane < n) {
       float kx[NS], ky[NS], kz[NS];
       const int s0 = start_and_kernel<float, NS>(x, ctx.nf[0], kx, ctx.kp);
@@ -749,37 +752,47 @@ __device__ __forceinline__ void col_spread_range(const SpreadCtx<float> &ctx,
       float wz8[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) wz8[m] = m < NS ? kz[m] : 0.f;
-      float4 *r = reinterpret_cast<float4 *>(stage + lane * COL_REC);
-      r[0] = make_float4(wz8[0], wz8[1], wz8[2], wz8[3]);
-      r[1] = make_float4(wz8[4], wz8[5], wz8[6], wz8[7]);
+      rec[0] = make_float4(wz8[0], wz8[1], wz8[2], wz8[3]);
+      rec[1] = make_float4(wz8[4], wz8[5], wz8[6], wz8[7]);
 #pragma unroll
       for (int t = 0; t < 8; t += 2)
-        r[2 + t / 2] = make_float4(cv.x * wx8[t], cv.y * wx8[t], cv.x * wx8[t + 1],
-                                   cv.y * wx8[t + 1]);
-      r[6] = make_float4(wy8[0], wy8[1], wy8[2], wy8[3]);
-      r[7] = make_float4(wy8[4], wy8[5], wy8[6], wy8[7]);
+        rec[2 + t / 2] = make_float4(cv.x * wx8[t], cv.y * wx8[t], cv.x * wx8[t + 1],
+                                     cv.y * wx8[t + 1]);
+      rec[6] = make_float4(wy8[0], wy8[1], wy8[2], wy8[3]);
+      rec[7] = make_float4(wy8[4], wy8[5], wy8[6], wy8[7]);
     }
-    if (stager && p + COL_BATCH + lane < u1) fc = c[fj];  // the next batch's strengths
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-
-    int k = 0;
-    while (k < n) {
-      const int sc = __builtin_amdgcn_readlane(posv, k) >> 6;
-      if (sc != cur) {  // another sub-column: hand the columns to the tile
-        if (cur >= 0) col_flush<ZD, P * P>(ar, ai, re, im, colcell, zlo, zhi, on);
-        cur = sc;
-        zlo = ZD;
-        zhi = 0;
-        colcell = ((sc / NSX) * S + lj) * P + (sc % NSX) * S + li;
+    if (p + 64 + lane < u1) fc = c[fj];  // the next pass's strengths
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kb = h * COL_BATCH;
+      const int nh = min(COL_BATCH, n - kb);
+      if (nh <= 0) break;
+      if (lane >= kb && lane < kb + nh) {
+        float4 *r = reinterpret_cast<float4 *>(stage + (lane - kb) * COL_REC);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[q] = rec[q];
       }
-      col_walk<NS, ZD, 0>(k, n, 64 * sc, posv, stage, xoff, yoff, ar, ai, zlo, zhi);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+      int k = 0;
+      while (k < nh) {
+        const int sc = __builtin_amdgcn_readlane(posv, kb + k) >> 6;
+        if (sc != cur) {  // another sub-column: hand the columns to the tile
+          if (cur >= 0) col_flush<ZD, P * P>(ar, ai, re, im, colcell, zlo, zhi, on);
+          cur = sc;
+          zlo = ZD;
+          zhi = 0;
+          colcell = ((sc / NSX) * S + lj) * P + (sc % NSX) * S + li;
+        }
+        col_walk<NS, ZD, 0>(k, nh, kb, 64 * sc, posv, stage, xoff, yoff, ar, ai, zlo, zhi);
+      }
+      // all lanes are done reading these records before restaging
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    // all lanes are done reading this batch's records before restaging
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
   if (cur >= 0) col_flush<ZD, P * P>(ar, ai, re, im, colcell, zlo, zhi, on);
 }
@@ -810,7 +823,7 @@ __device__ __forceinline__ void col_spread_unit(const SpreadCtx<float> &ctx,
 // Whole-bin column spreading in parity colors with a plain tile flush; bins
 // with more than dense_min points are left to spread_col_sub_kernel.
 template <int NS>
-__global__ void __launch_bounds__(64 * COL_WAVES)
+__global__ void __launch_bounds__(64 * COL_WAVES, 4)
     spread_col_colored_kernel(SpreadCtx<float> ctx, const float2 *__restrict__ c,
                               float2 *__restrict__ fw, int64_t nftot, int col0, int col1,
                               int col2) {
@@ -829,7 +842,7 @@ __global__ void __launch_bounds__(64 * COL_WAVES)
 
 // Column spreading of the dense bins' subproblems with an atomic tile flush.
 template <int NS>
-__global__ void __launch_bounds__(64 * COL_WAVES)
+__global__ void __launch_bounds__(64 * COL_WAVES, 4)
     spread_col_sub_kernel(SpreadCtx<float> ctx, const float2 *__restrict__ c,
                           float2 *__restrict__ fw, int64_t nftot) {
   extern __shared__ __align__(16) unsigned char lds_raw[];

@@ -227,6 +227,22 @@ def test_column_spreader_vs_direct(tmp_path_factory, tmp_path, geom, var, maxsub
         assert err < 2e-6
 
 
+@pytest.mark.parametrize("ns", [4, 7])
+def test_column_spreader_many_points_per_wave(tmp_path_factory, tmp_path, ns):
+    """Whole bins of about 470 points: every wave walks several passes of 64
+    points, each as two staged halves of 32 (the other tests keep waves
+    below one half)."""
+    nf, bs = (32, 32, 24), (8, 8, 6)
+    exe = _exe(tmp_path_factory, "float", 3, ns)
+    nsub, xs, c, fw = _run(exe, tmp_path, "col", nf, bs, 30000, 1024, 1, 0, 15)
+    idx, w = _weights(xs, nf, ns, 2.30 * ns)
+    ref = np.zeros(int(np.prod(nf)), complex)
+    np.add.at(ref, idx.ravel(), (w * c[0][:, None]).ravel())
+    err = np.linalg.norm(fw[0] - ref) / np.linalg.norm(ref)
+    print(ns, f"relerr {err:.2e}")
+    assert err < 2e-6
+
+
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
 def test_interp_grad_kernel_vs_direct(tmp_path_factory, tmp_path, geom):
     """Values and coordinate derivatives of the interpolation (the coordinate
