@@ -474,7 +474,9 @@ void Plan<T>::setup_bins() {
     if (user_bins) {
       for (int d = 0; d < 3; ++d)
         bs[d] = d < dim_ ? (user[d] > 0 ? user[d] : std::min(8, nf_[d])) : 1;
-      ok = size_t(tile_cells(dim_, bs, ns_)) * cell <= size_t(lds_limit);
+      // (interpolation tiles leave room for the bank-ordered dealing's bounds)
+      ok = size_t(tile_cells(dim_, bs, ns_)) * cell + (spreads ? 0 : INTERP_BANK_LDS) <=
+           size_t(lds_limit);
     } else {
       // about 1024 workgroups per launch (bins x vectors of a batch)
       const int64_t min_units = (1024 + batchsize_ - 1) / batchsize_;
@@ -567,6 +569,7 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
   binstart_.reserve_at_least(size_t(nb) + 1);
   const T *pts[3] = {xyz_[0], xyz_[1], xyz_[2]};
   if (geom_.colS) colstart_.reserve_at_least(size_t(nb) * geom_.nsub() + 1);
+  if (geom_.bank) colstart_.reserve_at_least(size_t(nb) * 32 + 1);
   if (opts_.gpu_method == 1 && !opts_.gpu_sort) {
     iota(idx_.get(), M_, stream_);
   } else {
@@ -578,7 +581,7 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
       sorted[d] = sxyz_[d].get();
     }
     bin_sort<T>(geom_, M_, pts, idx_.get(), binstart_.get(), scratch_, stream_,
-                geom_.colS ? colstart_.get() : nullptr, sorted);
+                (geom_.colS || geom_.bank) ? colstart_.get() : nullptr, sorted);
   }
   if (opts_.gpu_method != 1) {
     substart_.reserve_at_least(size_t(nb) + 1);
@@ -750,6 +753,7 @@ SpreadCtx<T> Plan<T>::spread_ctx() const {
   c.colored = colored_ ? 1 : 0;
   c.colS = geom_.colS;
   c.colstart = colstart_.get();
+  c.bank = geom_.bank;
   c.dense_min = dense_min_;
   c.n_dense_sub = n_dense_sub_;
   c.od_np = od_np_;

@@ -441,6 +441,11 @@ void bin_sort(const BinGeom &g, int M, const T *const xyz[3], int *idx, int *bin
     bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(
         kin, M, nb, banks ? 32 : 1, binstart);
     launch_check("bin_bounds_kernel");
+    if (banks && colstart) {  // the start of every (bin, bank class)
+      bin_bounds_kernel<<<blocks_for(int64_t(M) + 1, 256), 256, 0, stream>>>(kin, M, nb * 32, 1,
+                                                                           colstart);
+      launch_check("bin_bounds_kernel");
+    }
   }
   if (sorted) {
     if (g.dim == 1) {

@@ -59,7 +59,8 @@ struct SortScratch {
 
 // idx[M]: sorted point order. binstart[nbins+1]: first sorted position of each
 // bin, binstart[nbins] = M. With g.colS, colstart[nbins * nsub + 1] receives
-// the first sorted position of each (bin, sub-column). sorted[d] (M values
+// the first sorted position of each (bin, sub-column); with g.bank (and a
+// non-null colstart), colstart[nbins * 32 + 1] that of each (bin, bank class). sorted[d] (M values
 // each, d < dim; or null): the coordinates in the sorted order. Nothing is
 // synchronized with the host.
 template <typename T>

@@ -1008,18 +1008,70 @@ template <typename T, int DIM, int NS>
 __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const StartTile<DIM, NS> &st,
                                             cplx_t<T> *tile, const cplx_t<T> *__restrict__ g,
                                             cplx_t<T> *__restrict__ cout) {
+  // Bank-ordered units: the unit's points are sorted by the bank class r of
+  // their first tile cell (cell mod 32), and cls[r] .. cls[r + 1] are the
+  // positions of class r in the unit. Rounds j < R (R the smallest class
+  // count) deal point cls[r] + j to work item 32 j + r: the 32 lanes of a
+  // half-wave start in 32 different classes, so every gather of the half-wave
+  // hits distinct banks. The points beyond the full rounds (left[r] of class
+  // r, lp[r] their prefix) are dealt strided (below).
+  int *cls = nullptr, *lp = nullptr;
+  int R = 0, nleft = st.npts;
+  if (ctx.bank) {
+    cls = reinterpret_cast<int *>(tile + st.ntile);
+    lp = cls + 33;
+    if (threadIdx.x <= 32) {
+      const int v = ctx.colstart[int64_t(st.bin) * 32 + threadIdx.x];
+      cls[threadIdx.x] = min(max(v, st.first), st.first + st.npts);
+    }
+  }
   grid_to_tile<DIM, BIN_THREADS>(ctx, st.P, st.off, st.ntile, g, tile);
   __syncthreads();
-  // Strided dealing: work item i = 32 g + m takes the unit's point m G + g
-  // (G = ceil(npts / 32)). When the bin is ordered by LDS bank class (the
-  // sort's bank refinement), the 32 lanes of a half-wave then start their
-  // gathers in 32 different bank classes, in class order. (Dealing classes
-  // 4 apart to each 8 consecutive lanes instead doubled the kernel's bank
-  // conflict cycles.)
-  const int G = (st.npts + 31) >> 5;
-  for (int i = threadIdx.x; i < 32 * G; i += BIN_THREADS) {
-    const int pos = (i & 31) * G + (i >> 5);
-    if (pos >= st.npts) continue;
+  if (ctx.bank) {
+    if (threadIdx.x == 0) {
+      int r0 = st.npts;
+      for (int r = 0; r < 32; ++r) r0 = min(r0, cls[r + 1] - cls[r]);
+      int acc = 0;
+      for (int r = 0; r < 32; ++r) {
+        lp[r] = acc;
+        acc += cls[r + 1] - cls[r] - r0;
+      }
+      lp[32] = acc;
+      lp[33] = r0;
+    }
+    __syncthreads();
+    R = lp[33];
+    nleft = lp[32];
+  }
+  // Strided dealing of the remaining points: work item i = 32 g + m takes
+  // remaining point m G + g (G = ceil(nleft / 32)). In a bank-ordered unit
+  // these are in class order too, so the 32 lanes of a half-wave mostly
+  // start in different classes.
+  const int nfull = 32 * R;
+  const int G = (nleft + 31) >> 5;
+  for (int i = threadIdx.x; i < nfull + 32 * G; i += BIN_THREADS) {
+    int pos;
+    if (i < nfull) {
+      pos = cls[i & 31] + (i >> 5) - st.first;
+    } else {
+      const int t = i - nfull;
+      const int u = (t & 31) * G + (t >> 5);
+      if (u >= nleft) continue;
+      if (R > 0) {
+        // the class holding remaining point u: the last r with lp[r] <= u
+        int lo = 0, hi = 31;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (lp[mid] <= u)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        pos = cls[lo] + R + (u - lp[lo]) - st.first;
+      } else {
+        pos = u;
+      }
+    }
     const int q = st.first + pos;
     const int j = ctx.idx[q];
     T ker[DIM][NS];
@@ -1315,10 +1367,11 @@ void interp_ns(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int b
     launch_check("interp_gm_kernel");
   } else {
     auto k = interp_bin_kernel<T, DIM, NS>;
-    allow_lds(k, ctx.lds_bytes);
+    const size_t lds = ctx.lds_bytes + (ctx.bank ? INTERP_BANK_LDS : 0);
+    allow_lds(k, lds);
     dim3 grid(ctx.interp_sub ? ctx.sub_capacity : ctx.nbins[0] * ctx.nbins[1] * ctx.nbins[2],
               blk);
-    k<<<grid, BIN_THREADS, ctx.lds_bytes, stream>>>(ctx, c, fw, nftot);
+    k<<<grid, BIN_THREADS, lds, stream>>>(ctx, c, fw, nftot);
     launch_check("interp_bin_kernel");
   }
 }

@@ -73,6 +73,10 @@ struct SpreadCtx {
 
   int colS = 0;
   const int *colstart = nullptr;
+  // Interpolation plans: each bin is ordered by the LDS bank class of the
+  // points' first tile cells (32 classes), and colstart holds the sorted
+  // start of every (bin, class) (nbins_tot * 32 + 1 entries).
+  int bank = 0;
   int dense_min = 0;
   int n_dense_sub = 0;
 
@@ -81,6 +85,10 @@ struct SpreadCtx {
   int od_np = 0;
   size_t lds_od = 0;
 };
+
+// LDS bytes after the tile of a bank-ordered interpolation unit: its class
+// bounds, leftover prefix and full rounds (67 ints).
+constexpr size_t INTERP_BANK_LDS = 512;
 
 // Staging bytes of the output-driven spreader for np points: strengths (re
 // and im arrays), kernel values (dim * ns per point) and first tile cells

@@ -64,7 +64,10 @@ int main(int argc, char **argv) {
   // Column spreading refines the key inside a bin to (sub-column of S x S
   // first nodes in x, y; local z of the first node), the library's sort key.
   const int S = col ? col_side(NS) : 1, NSX = col ? 8 / S : 1, NSUB = NSX * NSX;
-  const int nz = col ? bs[2] : 1;
+  // Bank-ordered interpolation (bbin, bsub): the key inside a bin is the LDS
+  // bank class of the first tile cell, (lx + P0 ly + P0 P1 lz) mod 32.
+  const bool bank = var == "bbin" || var == "bsub";
+  const int nz = col ? bs[2] : (bank ? 32 : 1);
   const int nkeys = nbt * NSUB * nz;
   std::vector<int> key(M), kcnt(nkeys + 1, 0);
   const int fold64 = sizeof(T) == 4 ? 1 : 0;
@@ -79,6 +82,11 @@ int main(int argc, char **argv) {
       stride *= nb[d];
     }
     if (col) k = (k * NSUB + (loc[0] / S) + NSX * (loc[1] / S)) * nz + loc[2];
+    if (bank) {
+      const int P0 = bs[0] + NS - 1, P1 = DIM > 1 ? bs[1] + NS - 1 : 1;
+      const int cell = loc[0] + P0 * (loc[1] + P1 * loc[2]);
+      k = k * 32 + (cell & 31);
+    }
     key[i] = k;
     ++kcnt[k + 1];
   }
@@ -86,12 +94,15 @@ int main(int argc, char **argv) {
   std::vector<int> cnt(nbt + 1);
   for (int b = 0; b <= nbt; ++b) cnt[b] = kcnt[int64_t(b) * NSUB * nz];
   int *idx = alloc<int>(M), *binstart = alloc<int>(nbt + 1);
-  int *colstart = alloc<int>(size_t(nbt) * NSUB + 1);
+  int *colstart = alloc<int>(size_t(nbt) * (bank ? 32 : NSUB) + 1);
   {
     std::vector<int> pos(kcnt.begin(), kcnt.end() - 1);
     for (int i = 0; i < M; ++i) idx[pos[key[i]]++] = i;
     std::copy(cnt.begin(), cnt.end(), binstart);
-    for (int q = 0; q <= nbt * NSUB; ++q) colstart[q] = kcnt[int64_t(q) * nz];
+    if (bank)
+      std::copy(kcnt.begin(), kcnt.end(), colstart);  // (bin, class) starts
+    else
+      for (int q = 0; q <= nbt * NSUB; ++q) colstart[q] = kcnt[int64_t(q) * nz];
   }
   // Column spreading: colored launches take the bins of at most dense_min
   // points; the subproblem table covers the denser bins only.
@@ -140,7 +151,11 @@ int main(int argc, char **argv) {
   for (int d = 0; d < DIM; ++d) ntile *= bs[d] + NS - 1;
   ctx.lds_bytes = size_t(ntile) * 2 * sizeof(T);
   ctx.lds_spread = size_t(ntile) * 2 * sizeof(double);
-  ctx.interp_sub = var == "binsub" ? 1 : 0;
+  ctx.interp_sub = var == "binsub" || var == "bsub" ? 1 : 0;
+  if (bank) {
+    ctx.bank = 1;
+    ctx.colstart = colstart;
+  }
   if (var == "od") {
     const char *e = std::getenv("EMU_NP");
     ctx.od_np = e ? std::atoi(e) : 64;
@@ -210,8 +225,9 @@ int main(int argc, char **argv) {
       }
     };
     launch_col(ctx, c, fw);
-  } else if (var == "bin" || var == "binsub") {
-    emu_launch(var == "bin" ? nbt : cap, BIN_THREADS, ctx.lds_bytes,
+  } else if (var == "bin" || var == "binsub" || bank) {
+    emu_launch(var == "bin" || var == "bbin" ? nbt : cap, BIN_THREADS,
+               ctx.lds_bytes + (bank ? INTERP_BANK_LDS : 0),
                [&] { interp_bin_kernel<T, DIM, NS>(ctx, c, fw, nftot); }, blk);
   } else if (var == "mv") {
     // the batch path's multi-vector interpolation: two problems (the same

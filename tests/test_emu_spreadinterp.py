@@ -183,7 +183,7 @@ def test_output_driven_spread_vs_direct(tmp_path_factory, tmp_path, geom, np_od)
 
 
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
-@pytest.mark.parametrize("var", ["igm", "bin", "binsub", "mv"])
+@pytest.mark.parametrize("var", ["igm", "bin", "binsub", "bbin", "bsub", "mv"])
 def test_interp_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
     real, dim, ns, nf, bs, M, maxsub, blk, clustered = geom
     exe = _exe(tmp_path_factory, real, dim, ns)
@@ -241,6 +241,21 @@ def test_column_spreader_many_points_per_wave(tmp_path_factory, tmp_path, ns):
     err = np.linalg.norm(fw[0] - ref) / np.linalg.norm(ref)
     print(ns, f"relerr {err:.2e}")
     assert err < 2e-6
+
+
+@pytest.mark.parametrize("var,maxsub", [("bbin", 1024), ("bsub", 700), ("bsub", 64)])
+def test_bank_ordered_interp_dense_bins(tmp_path_factory, tmp_path, var, maxsub):
+    """Bank-ordered interpolation on bins of about 1900 points: full rounds of
+    one point per bank class, then the remaining points strided; whole bins
+    (bbin) and subproblems whose class bounds are clipped to the unit (bsub)."""
+    nf, bs = (32, 32, 32), (16, 8, 8)
+    exe = _exe(tmp_path_factory, "float", 3, 7)
+    nsub, xs, c, fw = _run(exe, tmp_path, var, nf, bs, 60000, maxsub, 1, 0, 16)
+    idx, w = _weights(xs, nf, 7, 2.30 * 7)
+    ref = (fw[0][idx] * w).sum(axis=1)
+    err = np.linalg.norm(c[0] - ref) / np.linalg.norm(ref)
+    print(var, maxsub, f"subproblems {nsub} relerr {err:.2e}")
+    assert np.isfinite(c[0]).all() and err < 2e-6
 
 
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
