@@ -364,12 +364,24 @@ def c3_comparison(pts, c, f, shape, tol, steps, err_samples):
             p2.execute(f, out=c_out)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) / steps * 1e3
+        # stage split from one more (untimed) step with HIP events
+        p1.set_stage_timing(True)
+        p2.set_stage_timing(True)
+        p1.stage_times()
+        p2.stage_times()
+        p1.execute(c, out=fk_out)
+        p2.execute(f, out=c_out)
+        st1 = {k: round(v, 3) for k, v in p1.stage_times().items() if v > 0}
+        st2 = {k: round(v, 3) for k, v in p2.stage_times().items() if v > 0}
+        p1.set_stage_timing(False)
+        p2.set_stage_timing(False)
         e1 = acc.relerr(fk_out.reshape(-1)[midx.to(dev)], ref1)
         e2 = acc.relerr(c_out[pidx.to(dev)], ref2)
         i1, i2 = p1.info(), p2.info()
         out[name] = {
             "ms_per_step": round(ms, 3), "nu_pts_per_s": 2 * M / (ms * 1e-3),
             "error": {"type1": e1, "type2": e2},
+            "stages_ms": {"type1": st1, "type2": st2},
             "type1": {"options": model.arms[a1] if (model and a1 is not None) else {},
                       "ns": i1["ns"], "gpu_method": i1["gpu_method"],
                       "binsize": i1["binsize"], "upsampfac": i1.get("upsampfac")},

@@ -591,7 +591,12 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
     dense_min_ = geom_.colS && colored_ ? 8 * opts_.gpu_maxsubprobsize : 0;
     // Interpolation plans cut bins into subproblems of sub_size_ points (one
     // per bin for typical densities, several for dense bins).
-    sub_size_ = type_ == 2 ? 4 * opts_.gpu_maxsubprobsize : opts_.gpu_maxsubprobsize;
+    // Column spreading cuts its dense bins into subproblems of dense_min_
+    // points: each subproblem adds its whole padded tile to the grid with
+    // atomics, and the dense centre of radial spokes (bins of 1e5 points)
+    // otherwise sends about a hundred tiles' atomics to the same cells.
+    sub_size_ = type_ == 2 ? 4 * opts_.gpu_maxsubprobsize
+                           : (dense_min_ > 0 ? dense_min_ : opts_.gpu_maxsubprobsize);
     sub_capacity_ = subproblem_capacity(M_, nb, sub_size_);
     sub2bin_.reserve_at_least(std::max(sub_capacity_, 1));
     build_subproblems(binstart_.get(), nb, sub_size_, dense_min_, substart_.get(),

@@ -1017,6 +1017,7 @@ __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const Start
   // r, lp[r] their prefix) are dealt strided (below).
   int *cls = nullptr, *lp = nullptr;
   int R = 0, nleft = st.npts;
+  bool contiguous = false;
   if (ctx.bank) {
     cls = reinterpret_cast<int *>(tile + st.ntile);
     lp = cls + 33;
@@ -1029,8 +1030,11 @@ __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const Start
   __syncthreads();
   if (ctx.bank) {
     if (threadIdx.x == 0) {
-      int r0 = st.npts;
-      for (int r = 0; r < 32; ++r) r0 = min(r0, cls[r + 1] - cls[r]);
+      int r0 = st.npts, occupied = 0;
+      for (int r = 0; r < 32; ++r) {
+        r0 = min(r0, cls[r + 1] - cls[r]);
+        occupied += cls[r + 1] > cls[r] ? 1 : 0;
+      }
       int acc = 0;
       for (int r = 0; r < 32; ++r) {
         lp[r] = acc;
@@ -1038,15 +1042,22 @@ __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const Start
       }
       lp[32] = acc;
       lp[33] = r0;
+      lp[34] = r0 == 0 && occupied < 16 ? 1 : 0;
     }
     __syncthreads();
     R = lp[33];
     nleft = lp[32];
+    contiguous = lp[34] != 0;
   }
   // Strided dealing of the remaining points: work item i = 32 g + m takes
   // remaining point m G + g (G = ceil(nleft / 32)). In a bank-ordered unit
   // these are in class order too, so the 32 lanes of a half-wave mostly
-  // start in different classes.
+  // start in different classes. A unit whose points fall in fewer than 16
+  // classes (a tight cluster, such as the centre of radial spokes) is dealt
+  // contiguously instead: strided lanes would take points of one class in
+  // different cells (the same bank at different addresses), while
+  // consecutive points of the sorted order (stable within a class) mostly
+  // share their cells, whose reads the LDS broadcasts.
   const int nfull = 32 * R;
   const int G = (nleft + 31) >> 5;
   for (int i = threadIdx.x; i < nfull + 32 * G; i += BIN_THREADS) {
@@ -1055,7 +1066,7 @@ __device__ __forceinline__ void interp_unit(const SpreadCtx<T> &ctx, const Start
       pos = cls[i & 31] + (i >> 5) - st.first;
     } else {
       const int t = i - nfull;
-      const int u = (t & 31) * G + (t >> 5);
+      const int u = contiguous ? t : (t & 31) * G + (t >> 5);
       if (u >= nleft) continue;
       if (R > 0) {
         // the class holding remaining point u: the last r with lp[r] <= u

@@ -87,7 +87,7 @@ struct SpreadCtx {
 };
 
 // LDS bytes after the tile of a bank-ordered interpolation unit: its class
-// bounds, leftover prefix and full rounds (67 ints).
+// bounds, leftover prefix, full rounds and dealing mode (68 ints).
 constexpr size_t INTERP_BANK_LDS = 512;
 
 // Staging bytes of the output-driven spreader for np points: strengths (re
