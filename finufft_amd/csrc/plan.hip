@@ -588,7 +588,10 @@ void Plan<T>::sort_points(const T *x, const T *y, const T *z) {
     // Column spreading runs whole bins in parity colors and only the bins
     // denser than dense_min_ (all bins when the shape is not colorable) as
     // subproblems; their count is read back so that the launch is exact.
-    dense_min_ = geom_.colS && colored_ ? 8 * opts_.gpu_maxsubprobsize : 0;
+    // The limit is one maxsub: a few whole-bin workgroups of far more points
+    // than the typical bin (the centre of radial spokes) would hold back
+    // their color's launch.
+    dense_min_ = geom_.colS && colored_ ? opts_.gpu_maxsubprobsize : 0;
     // Interpolation plans cut bins into subproblems of sub_size_ points (one
     // per bin for typical densities, several for dense bins).
     // Column spreading cuts its dense bins into subproblems of dense_min_

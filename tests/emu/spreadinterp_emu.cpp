@@ -106,7 +106,7 @@ int main(int argc, char **argv) {
   }
   // Column spreading: colored launches take the bins of at most dense_min
   // points; the subproblem table covers the denser bins only.
-  const int dense_min = var == "col" ? M : var == "colmix" ? 8 * maxsub : 0;
+  const int dense_min = var == "col" ? M : var == "colmix" ? maxsub : 0;
   int *substart = alloc<int>(nbt + 1);
   substart[0] = 0;
   for (int b = 0; b < nbt; ++b) {

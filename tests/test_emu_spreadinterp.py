@@ -210,7 +210,7 @@ COL_GEOMS = [
                                                   ("colmix", 8, 1)])
 def test_column_spreader_vs_direct(tmp_path_factory, tmp_path, geom, var, maxsub, clustered):
     """Whole bins in parity colors (col), every bin as subproblems (colsub),
-    and the mix the library runs on clustered points: bins above 8 maxsub
+    and the mix the library runs on clustered points: bins above maxsub
     points as subproblems, the rest colored (colmix)."""
     ns, nf, bs = geom
     exe = _exe(tmp_path_factory, "float", 3, ns)

@@ -278,7 +278,7 @@ def test_column_spread_not_colorable(torch_gpu):
 @pytest.mark.parametrize("maxsub", [1024, 64])
 def test_column_spread_dense_bins(torch_gpu, maxsub):
     """A tight cluster puts more points in one bin than a whole-bin
-    workgroup takes (8 maxsub): those bins run as subproblems after the
+    workgroup takes (maxsub): those bins run as subproblems after the
     colored launches."""
     r = rng(23)
     shape = (24, 24, 24)
