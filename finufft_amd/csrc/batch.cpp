@@ -15,14 +15,22 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
   for (const Plan<T> *p : plans_)
     if (p == nullptr) fail(FINUFFT_ERR_PLAN_NOTVALID, "batch: null plan");
   const Plan<T> &p0 = *plans_[0];
+  type_ = p0.type_;
   dim_ = p0.dim_;
   ntr_ = p0.ntransf_;
   iflag_ = p0.iflag_;
   ns_ = p0.ns_;
   for (const Plan<T> *p : plans_) {
-    if (p->type_ != 2 || p->spread_only_ || p->opts_.gpu_method == 1)
+    if ((p->type_ != 1 && p->type_ != 2) || p->type_ != type_ || p->spread_only_ ||
+        p->opts_.gpu_method == 1)
       fail(FINUFFT_ERR_INVALID_ARGUMENT,
-           "batch: every plan must be a full type-2 transform with gpu_method 2");
+           "batch: the plans must be full transforms of one type (1 or 2) with gpu_method 2");
+    // type 1: the subproblem table of the plan must cover every bin (the
+    // column-window spreader of 3-D fp32 plans keeps only its dense bins)
+    if (p->type_ == 1 && (p->geom_.colS != 0 || p->opts_.gpu_method == 3))
+      fail(FINUFFT_ERR_INVALID_ARGUMENT,
+           "batch: type-1 plans of the column-window or output-driven spreaders are not "
+           "batched");
     if (p->dim_ != dim_ || p->ntransf_ != ntr_ || p->iflag_ != iflag_ || p->ns_ != ns_ ||
         p->opts_.upsampfac != p0.opts_.upsampfac ||
         p->opts_.gpu_kerevalmeth != p0.opts_.gpu_kerevalmeth)
@@ -78,7 +86,7 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
 
   // 2-D grids whose sides fit one LDS buffer are made by the fused column /
   // row transforms; otherwise by amplification and rocFFT.
-  fused_ = dim_ == 2;
+  fused_ = dim_ == 2 && type_ == 2;
   for (size_t gi = 0; gi < groups_.size() && fused_; ++gi) {
     Group &g = *groups_[gi];
     fused_ = fused_fft_shape<T>(g.nf, g.cw, g.lcw, g.rw, g.lrw, g.lds_col, g.lds_row);
@@ -129,6 +137,7 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
       c.group = gi;
       c.full = c.count == g.per_chunk;
       c.amp_blocks = c.units = c.col_blocks = c.row_blocks = 0;
+      c.cells = int64_t(c.count) * V_ * nft;
       chunks_.push_back(c);
       arena_cells = std::max<int64_t>(arena_cells, int64_t(c.count) * V_ * nft);
     }
@@ -168,7 +177,7 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
       it.nsub = nsub[i];
       it.unit_begin = units;
       units += nsub[i];
-      lds_bytes_ = std::max(lds_bytes_, p.lds_bytes_);
+      lds_bytes_ = std::max(lds_bytes_, type_ == 1 ? p.lds_spread_ : p.lds_bytes_);
 
       if (fused_) {
         FusedFftItem<T> &f = ffi[s];
@@ -223,9 +232,14 @@ BatchPlan<T>::BatchPlan(std::vector<Plan<T> *> plans, size_t chunk_bytes, hipStr
             "hipMemcpy(batch grids)");
   fk_host_.resize(n);
   c_host_.resize(n);
-  // Raise the interpolation kernel's LDS limit now, outside any capture.
-  interp_batch<T>(interp_items_.get(), n, dim_, ns_, kp_, fw_base_.get(), c_base_.get(), 0, V_,
-                  0, lds_bytes_, stream_);
+  // Raise the spreading / interpolation kernel's LDS limit now, outside any
+  // capture.
+  if (type_ == 1)
+    spread_batch<T>(interp_items_.get(), n, dim_, ns_, c_base_.get(), 0, fw_base_.get(), V_, 0,
+                    lds_bytes_, stream_);
+  else
+    interp_batch<T>(interp_items_.get(), n, dim_, ns_, kp_, fw_base_.get(), c_base_.get(), 0, V_,
+                    0, lds_bytes_, stream_);
   hip_check(hipStreamCreateWithFlags(&own_, hipStreamNonBlocking), "hipStreamCreate(batch)");
   hip_check(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming), "hipEventCreate(batch)");
   hip_check(hipEventCreateWithFlags(&ev_out_, hipEventDisableTiming), "hipEventCreate(batch)");
@@ -253,6 +267,20 @@ void BatchPlan<T>::enqueue(hipStream_t s) {
   for (int v0 = 0; v0 < ntr_; v0 += V_) {
     for (const Chunk &c : chunks_) {
       Group &g = *groups_[c.group];
+      if (type_ == 1) {
+        // clear the round's grids, spread every problem's points into them,
+        // transform, and read the modes out with the deconvolution
+        zero<T>(arena_.get(), c.cells, s);
+        spread_batch<T>(interp_items_.get() + c.first, c.count, dim_, ns_,
+                        c_base_.get() + c.first, v0, fw_base_.get() + c.first, V_, c.units,
+                        lds_bytes_, s);
+        (c.full ? g.fft_full : g.fft_rest).exec(arena_.get(), iflag_, s);
+        // type 1 writes the modes: the table of fk pointers is an output here
+        deconv_batch<T>(amp_items_.get() + c.first, fw_base_.get() + c.first,
+                        const_cast<C *const *>(fk_base_.get()) + c.first, c.count, dim_, v0,
+                        c.amp_blocks, s);
+        continue;
+      }
       if (fused_) {
         fused_fft_cols<T>(fft_items_.get() + c.first, fk_base_.get() + c.first,
                           fw_base_.get() + c.first, c.count, v0, iflag_, c.col_blocks, g.lds_col,

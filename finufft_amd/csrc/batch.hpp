@@ -74,6 +74,7 @@ class BatchPlan {
     int group;
     bool full;
     int64_t amp_blocks, units, col_blocks, row_blocks;
+    int64_t cells;  // grid cells of the chunk's round (type 1 clears them)
   };
   void enqueue(hipStream_t s);
 
@@ -84,7 +85,7 @@ class BatchPlan {
   // be captured), ordered after and before the caller's stream by events.
   hipStream_t own_ = nullptr;
   hipEvent_t ev_in_ = nullptr, ev_out_ = nullptr;
-  int dim_ = 0, ntr_ = 1, iflag_ = 1, ns_ = 2, V_ = 1;
+  int type_ = 2, dim_ = 0, ntr_ = 1, iflag_ = 1, ns_ = 2, V_ = 1;
   size_t lds_bytes_ = 0;
   KerParams<T> kp_{};
   dbuf<C> arena_;

@@ -37,6 +37,29 @@ void interp(const SpreadCtx<T> &ctx, cplx_t<T> *c, const cplx_t<T> *fw, int blk,
 }
 
 template <typename T>
+void spread_batch(const InterpItem<T> *items, int nprob, int dim, int ns,
+                  const cplx_t<T> *const *c_base, int64_t c_vec_offset, cplx_t<T> *const *fw_base,
+                  int nvec, int64_t total_units, size_t lds_bytes, hipStream_t stream) {
+  switch (dim) {
+    case 1:
+      return spread_batch_dim<T, 1>(items, nprob, ns, c_base, c_vec_offset, fw_base, nvec,
+                                    total_units, lds_bytes, stream);
+    case 2:
+      return spread_batch_dim<T, 2>(items, nprob, ns, c_base, c_vec_offset, fw_base, nvec,
+                                    total_units, lds_bytes, stream);
+    default:
+      return spread_batch_dim<T, 3>(items, nprob, ns, c_base, c_vec_offset, fw_base, nvec,
+                                    total_units, lds_bytes, stream);
+  }
+}
+template void spread_batch<float>(const InterpItem<float> *, int, int, int,
+                                  const cplx_t<float> *const *, int64_t, cplx_t<float> *const *,
+                                  int, int64_t, size_t, hipStream_t);
+template void spread_batch<double>(const InterpItem<double> *, int, int, int,
+                                   const cplx_t<double> *const *, int64_t,
+                                   cplx_t<double> *const *, int, int64_t, size_t, hipStream_t);
+
+template <typename T>
 void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns, const KerParams<T> &kp,
                   const cplx_t<T> *const *fw_base, cplx_t<T> *const *c_base,
                   int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,

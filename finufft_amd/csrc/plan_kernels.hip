@@ -202,6 +202,50 @@ __global__ void __launch_bounds__(EW_T)
   }
 }
 
+// The reverse of amplify_batch_kernel for type-1 batches: the mode cells of
+// every grid of the round, divided by the kernel's coefficients, to fk (the
+// blocks and their cells as there; cells outside the modes are skipped).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(EW_T)
+    deconv_batch_kernel(const AmpItem<T> *items, const cplx_t<T> *const *fw_base,
+                        cplx_t<T> *const *fk_base, int nprob, int64_t fk_vec_offset,
+                        int64_t total_blocks) {
+  const int64_t b = blockIdx.x;
+  if (b >= total_blocks) return;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].blk_begin <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const AmpItem<T> it = items[lo];
+  const gptr<const T> ker[3] = {to_global(it.ker[0]), to_global(it.ker[1]),
+                                to_global(it.ker[2])};
+  const int64_t local = b - it.blk_begin;
+  const int t = int(local / it.chunks);
+  const int g0 = int(local % it.chunks) * AMP_CHUNK;
+  const int g1 = g0 + AMP_CHUNK < it.nftot ? g0 + AMP_CHUNK : int(it.nftot);
+  const gptr<const vec2_t<T>> fw = to_global(reinterpret_cast<const vec2_t<T> *>(fw_base[lo])) +
+                                   int64_t(t) * it.nftot;
+  const gptr<vec2_t<T>> fk = to_global(reinterpret_cast<vec2_t<T> *>(fk_base[lo])) +
+                             (fk_vec_offset + t) * it.nmodes;
+  for (int o = int(threadIdx.x); o < g1 - g0; o += EW_T) {
+    const int g = g0 + o;
+    int kidx;
+    T kv;
+    if (cell_to_mode<T, DIM>(g, it.nf, it.rnf, it.ms, it.modeord, ker, kidx, kv)) {
+      const vec2_t<T> v = fw[g];
+      vec2_t<T> out;
+      out.x = v.x / kv;
+      out.y = v.y / kv;
+      fk[kidx] = out;
+    }
+  }
+}
+
 template <typename T>
 __global__ void zero_kernel(cplx_t<T> *p, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
@@ -350,6 +394,19 @@ void amplify_batch(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
 }
 
 template <typename T>
+void deconv_batch(const AmpItem<T> *items, const cplx_t<T> *const *fw_base,
+                  cplx_t<T> *const *fk_base, int nprob, int dim, int64_t fk_vec_offset,
+                  int64_t total_blocks, hipStream_t stream) {
+  if (total_blocks <= 0 || nprob <= 0) return;
+  if (total_blocks > INT32_MAX) fail(FINUFFT_ERR_MAXNALLOC, "deconv_batch: grid too large");
+  auto k = dim == 1 ? deconv_batch_kernel<T, 1>
+                    : (dim == 2 ? deconv_batch_kernel<T, 2> : deconv_batch_kernel<T, 3>);
+  k<<<unsigned(total_blocks), EW_T, 0, stream>>>(items, fw_base, fk_base, nprob, fk_vec_offset,
+                                                 total_blocks);
+  launch_check("deconv_batch_kernel");
+}
+
+template <typename T>
 void zero(cplx_t<T> *p, int64_t n, hipStream_t stream) {
   if (n <= 0) return;
   zero_kernel<T><<<ew_grid(n), EW_T, 0, stream>>>(p, n);
@@ -421,6 +478,8 @@ void cmul(const cplx_t<T> *a, const cplx_t<T> *b, cplx_t<T> *out, int n, int blk
   template void zero<T>(cplx_t<T> *, int64_t, hipStream_t);                                    \
   template void amplify_batch<T>(const AmpItem<T> *, const cplx_t<T> *const *,                 \
                                  cplx_t<T> *const *, int, int, int64_t, int64_t, hipStream_t); \
+  template void deconv_batch<T>(const AmpItem<T> *, const cplx_t<T> *const *,           \
+                               cplx_t<T> *const *, int, int, int64_t, int64_t, hipStream_t); \
   template void minmax<T>(const T *, int, double *, double *, hipStream_t);                    \
   template void affine<T>(const T *, T *, int, double, double, hipStream_t);                   \
   template void t3_prephase<T>(int, const T *const[3], const double[3], int, int, cplx_t<T> *, \

@@ -52,6 +52,14 @@ void amplify_batch(const AmpItem<T> *items, const cplx_t<T> *const *fk_base,
                    cplx_t<T> *const *fw_base, int nprob, int dim, int64_t fk_vec_offset,
                    int64_t total_blocks, hipStream_t stream);
 
+// Type-1 counterpart of amplify_batch (same items and blocks): fk_base[p]
+// (vector fk_vec_offset of the round) receives the mode cells of the grids
+// fw_base[p] divided by the kernel's Fourier coefficients.
+template <typename T>
+void deconv_batch(const AmpItem<T> *items, const cplx_t<T> *const *fw_base,
+                  cplx_t<T> *const *fk_base, int nprob, int dim, int64_t fk_vec_offset,
+                  int64_t total_blocks, hipStream_t stream);
+
 // Sets n complex values to zero (graph-capture friendly, no memset node).
 template <typename T>
 void zero(cplx_t<T> *p, int64_t n, hipStream_t stream);

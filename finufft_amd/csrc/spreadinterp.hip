@@ -1191,6 +1191,48 @@ __global__ void __launch_bounds__(MV_THREADS)
       nvec);
 }
 
+// Spreading of many problems (see spread_batch): workgroup b takes work unit
+// b (a subproblem of one problem) and adds it to the problem's grid for
+// every vector of the round in turn (fp64 tile, atomic flush, as
+// spread_sub_kernel).
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(BIN_THREADS)
+    spread_batch_kernel(const InterpItem<T> *__restrict__ items, int nprob,
+                        const cplx_t<T> *const *__restrict__ c_base, int64_t c_vec_offset,
+                        cplx_t<T> *const *__restrict__ fw_base, int nvec, int64_t total_units) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *re = reinterpret_cast<double *>(lds_raw);
+  const int64_t u = blockIdx.x;
+  if (u >= total_units) return;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].unit_begin <= u)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const InterpItem<T> it = items[lo];
+  StartTile<DIM, NS> st;
+  if (!st.init_sub(it.ctx, int(u - it.unit_begin)) || st.npts == 0) return;
+  double *im = re + st.ntile;
+  const cplx_t<T> *c = c_base[lo] + c_vec_offset * int64_t(it.ctx.M);
+  cplx_t<T> *fw = fw_base[lo];
+  for (int t = 0; t < nvec; ++t) {
+    for (int n = threadIdx.x; n < st.ntile; n += BIN_THREADS) {
+      re[n] = 0.0;
+      im[n] = 0.0;
+    }
+    __syncthreads();
+    spread_into_tile<T, DIM, NS>(it.ctx, st, c + int64_t(t) * it.ctx.M, re, im);
+    __syncthreads();
+    tile_to_grid<DIM, BIN_THREADS, false>(it.ctx, st.P, st.off, st.ntile, re, im,
+                                          fw + int64_t(t) * it.nftot);
+    __syncthreads();  // the tile is read out before the next vector clears it
+  }
+}
+
 // Interpolant and coordinate derivatives, one lane per point (sorted order),
 // reading the fine grid from HBM: c = sum g kx ky kz and
 // dc/dx_d = sum g (d k_d / dx_d) prod_{e != d} k_e, where d k / dx =
@@ -1402,6 +1444,19 @@ void interp_batch_ns(const InterpItem<T> *items, int nprob, const KerParams<T> &
 }
 
 template <typename T, int DIM, int NS>
+void spread_batch_ns(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *c_base,
+                     int64_t c_vec_offset, cplx_t<T> *const *fw_base, int nvec,
+                     int64_t total_units, size_t lds_bytes, hipStream_t stream) {
+  auto k = spread_batch_kernel<T, DIM, NS>;
+  allow_lds(k, lds_bytes);
+  if (total_units <= 0) return;
+  if (total_units > INT32_MAX) fail(FINUFFT_ERR_MAXNALLOC, "spread_batch: too many work units");
+  k<<<unsigned(total_units), BIN_THREADS, lds_bytes, stream>>>(items, nprob, c_base, c_vec_offset,
+                                                                fw_base, nvec, total_units);
+  launch_check("spread_batch_kernel");
+}
+
+template <typename T, int DIM, int NS>
 void interp_grad_ns(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
                     int blk, hipStream_t stream) {
   if (ctx.M == 0) return;
@@ -1453,6 +1508,16 @@ struct InterpBatchOp {
 };
 
 template <typename T, int DIM, int NS>
+struct SpreadBatchOp {
+  static void go(const InterpItem<T> *items, int nprob, const cplx_t<T> *const *c_base,
+                 int64_t c_vec_offset, cplx_t<T> *const *fw_base, int nvec, int64_t total_units,
+                 size_t lds_bytes, hipStream_t s) {
+    spread_batch_ns<T, DIM, NS>(items, nprob, c_base, c_vec_offset, fw_base, nvec, total_units,
+                                lds_bytes, s);
+  }
+};
+
+template <typename T, int DIM, int NS>
 struct InterpGradOp {
   static void go(const SpreadCtx<T> &ctx, cplx_t<T> *c, cplx_t<T> *dc, const cplx_t<T> *fw,
                  int blk, hipStream_t s) {
@@ -1478,6 +1543,15 @@ void interp_batch_dim(const InterpItem<T> *items, int nprob, int ns, const KerPa
 }
 
 template <typename T, int DIM>
+void spread_batch_dim(const InterpItem<T> *items, int nprob, int ns,
+                      const cplx_t<T> *const *c_base, int64_t c_vec_offset,
+                      cplx_t<T> *const *fw_base, int nvec, int64_t total_units, size_t lds_bytes,
+                      hipStream_t stream) {
+  ns_dispatch<T, SpreadBatchOp>::run(ns, items, nprob, c_base, c_vec_offset, fw_base, nvec,
+                                     total_units, lds_bytes, stream);
+}
+
+template <typename T, int DIM>
 void spread_dim(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
                 hipStream_t stream) {
   ns_dispatch<T, SpreadOp>::run(ctx.ns, ctx, c, fw, blk, stream);
@@ -1493,6 +1567,10 @@ template void spread_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, const cplx
                                           cplx_t<FA_REAL> *, int, hipStream_t);
 template void interp_dim<FA_REAL, FA_DIM>(const SpreadCtx<FA_REAL> &, cplx_t<FA_REAL> *,
                                           const cplx_t<FA_REAL> *, int, hipStream_t);
+template void spread_batch_dim<FA_REAL, FA_DIM>(const InterpItem<FA_REAL> *, int, int,
+                                                const cplx_t<FA_REAL> *const *, int64_t,
+                                                cplx_t<FA_REAL> *const *, int, int64_t, size_t,
+                                                hipStream_t);
 template void interp_batch_dim<FA_REAL, FA_DIM>(const InterpItem<FA_REAL> *, int, int,
                                                 const KerParams<FA_REAL> &,
                                                 const cplx_t<FA_REAL> *const *,

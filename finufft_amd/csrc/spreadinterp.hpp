@@ -179,7 +179,23 @@ void interp_batch(const InterpItem<T> *items, int nprob, int dim, int ns, const 
                   int64_t c_vec_offset, int nvec, int64_t total_units, size_t lds_bytes,
                   hipStream_t stream);
 
+// Spreads all problems of a launch, one workgroup per work unit (a
+// subproblem u - unit_begin of problem p; substart / sub2bin cover every
+// bin), for each of the nvec vectors of the round in turn: c_base[p] holds
+// the problem's strengths (vector c_vec_offset is the round's first),
+// fw_base[p] its vector-0 grid of the round (vectors nftot apart), which the
+// caller has cleared. lds_bytes: the largest fp64 spreading tile.
+template <typename T>
+void spread_batch(const InterpItem<T> *items, int nprob, int dim, int ns,
+                  const cplx_t<T> *const *c_base, int64_t c_vec_offset, cplx_t<T> *const *fw_base,
+                  int nvec, int64_t total_units, size_t lds_bytes, hipStream_t stream);
+
 // Per-dimension entry points (one translation unit each).
+template <typename T, int DIM>
+void spread_batch_dim(const InterpItem<T> *items, int nprob, int ns,
+                      const cplx_t<T> *const *c_base, int64_t c_vec_offset,
+                      cplx_t<T> *const *fw_base, int nvec, int64_t total_units, size_t lds_bytes,
+                      hipStream_t stream);
 template <typename T, int DIM>
 void spread_dim(const SpreadCtx<T> &ctx, const cplx_t<T> *c, cplx_t<T> *fw, int blk,
                 hipStream_t stream);

@@ -35,18 +35,20 @@ FINUFFT_EXPORT int cufinufft_interp_grad(cufinufft_plan d_plan, hipDoubleComplex
 FINUFFT_EXPORT int cufinufftf_interp_grad(cufinufftf_plan d_plan, hipFloatComplex *d_c,
                                           hipFloatComplex *d_dc, hipFloatComplex *d_fk);
 
-/* Fused execution of many independent type-2 problems. No reference
+/* Fused execution of many independent problems of one type. No reference
  * counterpart: the reference runs one plan per problem. plans[nplans] are
- * type-2 plans (cufinufft_plan or cufinufftf_plan as is_double says) with
- * points set, sharing dimension, ntransf, isign, tolerance and upsampfac,
- * made with gpu_method 0 or 2. Problems of equal fine-grid shape are run in
- * chunks whose grids fit arena_bytes (0: 192 MiB, a size meant to stay in the
- * GPU's last-level cache from stage to stage); the batch allocates one
- * chunk's grids. The plans must outlive the batch and keep their
- * points. execute takes host arrays of nplans device pointers: c[p] receives
- * the ntransf x M_p values of problem p, fk[p] holds its ntransf x N_p modes.
- * The whole schedule is recorded into a HIP graph on the first execute and
- * replayed afterwards, on the stream of plans[0]. */
+ * type-2 plans, or type-1 plans (not the 3-D fp32 column-window spreader's),
+ * (cufinufft_plan or cufinufftf_plan as is_double says) with points set,
+ * sharing type, dimension, ntransf, isign, tolerance and upsampfac, made with
+ * gpu_method 0 or 2. Problems of equal fine-grid shape are run in chunks
+ * whose grids fit arena_bytes (0: 192 MiB, a size meant to stay in the GPU's
+ * last-level cache from stage to stage); the batch allocates one chunk's
+ * grids. The plans must outlive the batch and keep their points. execute
+ * takes host arrays of nplans device pointers: c[p] holds the ntransf x M_p
+ * values of problem p (type 2: output; type 1: input), fk[p] its ntransf x
+ * N_p modes (type 2: input; type 1: output). The whole schedule is recorded
+ * into a HIP graph on the first execute and replayed afterwards, on the
+ * stream of plans[0]. */
 typedef void *finufft_amd_batch;
 FINUFFT_EXPORT int finufft_amd_batch_makeplan(int is_double, int nplans, void *const *plans,
                                               size_t arena_bytes, finufft_amd_batch *out);

@@ -90,8 +90,25 @@ int main(int argc, char **argv) {
   fwb[1] = fw_b2;
   emu_launch(blocks + 2, EW_T, 0,
              [&] { amplify_batch_kernel<T, DIM>(items, fkb, fwb, 2, 0, blocks); });
-  const bool same = std::memcmp(fw_b, fw_out, sizeof(C) * size_t(nftot) * blk) == 0 &&
-                    std::memcmp(fw_b2, fw_out, sizeof(C) * size_t(nftot) * blk) == 0;
+  bool same = std::memcmp(fw_b, fw_out, sizeof(C) * size_t(nftot) * blk) == 0 &&
+              std::memcmp(fw_b2, fw_out, sizeof(C) * size_t(nftot) * blk) == 0;
+  // the batched deconvolution (type-1 batches) of the same grids, twice in
+  // one launch, must give the modes of the per-plan deconvolution
+  C *fk_b1 = alloc<C>(size_t(nmodes) * blk), *fk_b2 = alloc<C>(size_t(nmodes) * blk);
+  for (int64_t i = 0; i < nmodes * blk; ++i) fk_b1[i] = fk_b2[i] = {T(NAN), T(NAN)};
+  const C **fwc = alloc<const C *>(2);
+  C **fkw = alloc<C *>(2);
+  fwc[0] = fwc[1] = fw_in;
+  fkw[0] = fk_b1;
+  fkw[1] = fk_b2;
+  emu_launch(blocks + 2, EW_T, 0,
+             [&] { deconv_batch_kernel<T, DIM>(items, fwc, fkw, 2, 0, blocks); });
+  same = same && std::memcmp(fk_b1, fk_out, sizeof(C) * size_t(nmodes) * blk) == 0 &&
+         std::memcmp(fk_b2, fk_out, sizeof(C) * size_t(nmodes) * blk) == 0;
+  std::free(fk_b1);
+  std::free(fk_b2);
+  std::free(fwc);
+  std::free(fkw);
 
   FILE *f = std::fopen(out, "wb");
   if (!f) return 2;

@@ -39,7 +39,8 @@ int main(int argc, char **argv) {
   const int clustered = std::atoi(argv[11]), seed = std::atoi(argv[12]);
   const char *out = argv[13];
   const bool col = var == "col" || var == "colsub" || var == "colmix";
-  const bool spreads = var == "gm" || var == "sub" || var == "colored" || var == "od" || col;
+  const bool spreads =
+      var == "gm" || var == "sub" || var == "colored" || var == "od" || var == "smv" || col;
   if (col && !(DIM == 3 && sizeof(T) == 4 && NS <= COL_MAX_NS && bs[0] == 8 && bs[1] == 8))
     return 3;
   if (var == "colored" || var == "col" || var == "colmix")
@@ -258,6 +259,43 @@ int main(int argc, char **argv) {
     std::free(fwb);
     std::free(cb);
     std::free(c2);
+    if (!same) return 4;
+  } else if (var == "smv") {
+    // the batch path's multi-problem spreading: two problems (the same
+    // points and strengths, separate grids) in one launch, all blk vectors
+    // per work unit; the second problem's grids must equal the first's
+    InterpItem<T> *items = alloc<InterpItem<T>>(2);
+    const C **cb = alloc<const C *>(2);
+    C **fwb = alloc<C *>(2);
+    C *fw2 = alloc<C>(size_t(nftot) * blk);
+    for (int64_t i = 0; i < nftot * blk; ++i) fw2[i] = {T(0), T(0)};
+    for (int q = 0; q < 2; ++q) {
+      items[q] = InterpItem<T>{};
+      items[q].ctx = ctx;
+      items[q].nftot = nftot;
+      items[q].nsub = nsub;
+      items[q].unit_begin = int64_t(q) * nsub;
+      cb[q] = c;
+    }
+    fwb[0] = fw;
+    fwb[1] = fw2;
+    const int64_t units = 2 * int64_t(nsub);
+    // two blocks past the end exercise the kernel's early return
+    emu_launch(units + 2, BIN_THREADS, ctx.lds_spread, [&] {
+      spread_batch_kernel<T, DIM, NS>(items, 2, cb, 0, fwb, blk, units);
+    });
+    // equal to rounding (the LDS tile sums its atomic adds in thread order)
+    double dmax = 0, amax = 0;
+    for (int64_t i = 0; i < nftot * blk; ++i) {
+      dmax = std::max({dmax, std::abs(double(fw[i].x - fw2[i].x)),
+                       std::abs(double(fw[i].y - fw2[i].y))});
+      amax = std::max({amax, std::abs(double(fw[i].x)), std::abs(double(fw[i].y))});
+    }
+    const bool same = dmax <= (sizeof(T) == 4 ? 1e-5 : 1e-12) * amax;
+    std::free(items);
+    std::free(cb);
+    std::free(fwb);
+    std::free(fw2);
     if (!same) return 4;
   } else if (var == "grad") {
     // values and coordinate derivatives of the interpolation (autograd)

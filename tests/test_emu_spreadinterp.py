@@ -144,7 +144,7 @@ GEOMS = [
 
 
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}-{g[1]}d-ns{g[2]}")
-@pytest.mark.parametrize("var", ["gm", "sub", "colored"])
+@pytest.mark.parametrize("var", ["gm", "sub", "colored", "smv"])
 def test_spread_kernels_vs_direct(tmp_path_factory, tmp_path, geom, var):
     real, dim, ns, nf, bs, M, maxsub, blk, clustered = geom
     if var == "colored" and not all(n % b == 0 and (n // b) % 2 == 0 and n // b >= 2

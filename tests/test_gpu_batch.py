@@ -1,4 +1,5 @@
-"""Fused execution of many uneven type-2 problems (finufft_amd.batch).
+"""Fused execution of many uneven type-2 (and type-1) problems
+(finufft_amd.batch).
 
 Every problem of the batch must give what its own plan gives (same kernels;
 the FFT runs batched over more vectors, so agreement is to rounding), and the
@@ -172,3 +173,65 @@ def test_batch_fused_transform_shapes(torch_gpu, dtype):
         for p, fk, o in zip(plans, fks, outs):
             ref = to_host(p.execute(fk))
             assert nudft.relerr(to_host(o), ref) < bound, (p.info()["nf"], modeord, isign)
+
+
+@pytest.mark.parametrize("dtype,dim", [("complex64", 2), ("complex128", 2), ("complex128", 3),
+                                       ("complex64", 1)])
+def test_batch_type1_matches_plans_and_direct(torch_gpu, dtype, dim):
+    """A batch of type-1 plans: every problem's modes equal its own plan's
+    (the same spreading and deconvolution, the FFT batched) and the direct
+    sum to 25 tol; several rounds of vectors (small arena) and both mode
+    orders."""
+    from finufft_amd import Plan
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    r = rng(120 + dim)
+    real = np.float32 if dtype == "complex64" else np.float64
+    tol = 1e-5 if dtype == "complex64" else 1e-11
+    ntr = 3
+    for modeord in (0, 1):
+        plans, pts, cs = [], [], []
+        for _ in range(9):
+            n = int(np.exp(r.uniform(np.log(8), np.log(40 if dim < 3 else 16))))
+            M = int(np.exp(r.uniform(np.log(100), np.log(3000))))
+            shape = (n * (8 if dim == 1 else 1),) * dim
+            x = [r.uniform(-np.pi, np.pi, M).astype(real) for _ in range(dim)]
+            p = Plan(1, shape, n_trans=ntr, eps=tol, dtype=dtype, modeord=modeord)
+            p.setpts(*[to_dev(torch, a) for a in x])
+            plans.append(p)
+            pts.append(x)
+            cs.append(to_dev(torch, cnormal(r, (ntr, M), np.dtype(dtype))))
+        outs = [torch.full((ntr,) + tuple(p.n_modes), float("nan"), dtype=c.dtype, device="cuda")
+                for p, c in zip(plans, cs)]
+        b = BatchPlan(plans, arena_bytes=1 << 22)  # small arena: several rounds and chunks
+        assert b.info()["nprob"] == 9
+        b.execute(cs, outs)
+        b.execute(cs, outs)  # the recorded graph, replayed
+        torch.cuda.synchronize()
+        for p, x, c, o in zip(plans, pts, cs, outs):
+            got = to_host(o)
+            assert np.isfinite(got).all()
+            ref_plan = to_host(p.execute(c))
+            assert nudft.relerr(got, ref_plan) < 1e-5
+            ch = to_host(c)
+            ref = np.stack([nudft.type1([a.astype(np.float64) for a in x], ch[t], p._isign,
+                                        tuple(p.n_modes), modeord=modeord) for t in range(ntr)])
+            err = nudft.relerr(got, ref)
+            assert err < 25 * tol, err
+
+
+def test_batch_type1_refuses_column_plans(torch_gpu):
+    """3-D fp32 type-1 plans spread with the column-window method, whose
+    subproblem table covers only its dense bins: the batch refuses them."""
+    from finufft_amd import FinufftError, Plan
+    from finufft_amd.batch import BatchPlan
+
+    torch = torch_gpu
+    r = rng(125)
+    x = [to_dev(torch, r.uniform(-np.pi, np.pi, 500).astype(np.float32)) for _ in range(3)]
+    p = Plan(1, (16, 16, 16), eps=1e-5, dtype="complex64")
+    p.setpts(*x)
+    assert p.info()["col_side"] > 0
+    with pytest.raises(FinufftError):
+        BatchPlan([p])
