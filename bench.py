@@ -153,6 +153,86 @@ def cpu_baseline(N, tol, M_full, M_sample, threads):
     }
 
 
+def batched_type1_comparison(nprob, ntr, tol, reps):
+    """The same uneven workload in the type-1 direction (strengths -> modes):
+    a loop of per-plan executes against the fused type-1 batch (per chunk one
+    spreading launch over all its problems, a batched FFT and one
+    deconvolution launch, replayed as a HIP graph); errors at sampled modes of
+    sampled problems against the FP64 direct sum."""
+    import numpy as np
+    import torch
+
+    from finufft_amd import Plan
+    from finufft_amd import accuracy as acc
+    from finufft_amd.batch import BatchPlan
+
+    dev = torch.device("cuda", 0)
+    r = np.random.default_rng(6)
+    Ns = np.rint(np.exp(r.uniform(np.log(32), np.log(512), nprob))).astype(int)
+    Ms = np.rint(np.exp(r.uniform(np.log(1e3), np.log(1e5), nprob))).astype(int)
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    plans, pts, cs, outs = [], [], [], []
+    for n, m in zip(Ns, Ms):
+        x = [torch.rand(int(m), generator=g, device=dev) * (2 * math.pi) - math.pi
+             for _ in range(2)]
+        c = torch.complex(torch.randn((ntr, int(m)), generator=g, device=dev),
+                          torch.randn((ntr, int(m)), generator=g, device=dev))
+        p = Plan(1, (int(n), int(n)), n_trans=ntr, eps=tol, dtype="complex64", isign=1)
+        p.setpts(*x)
+        plans.append(p)
+        pts.append(x)
+        cs.append(c)
+        outs.append(torch.empty((ntr, int(n), int(n)), dtype=torch.complex64, device=dev))
+    torch.cuda.synchronize()
+
+    def plain():
+        for p, c, o in zip(plans, cs, outs):
+            p.execute(c, out=o)
+
+    samp = r.choice(nprob, min(8, nprob), replace=False)
+    midx = {int(i): torch.randint(0, int(Ns[i]) ** 2, (32,), device=dev) for i in samp}
+
+    def errors():
+        e = 0.0
+        for i in samp:
+            n = int(Ns[i])
+            ref = acc.type1_at(pts[i], cs[i][0], 1, (n, n), midx[int(i)])
+            e = max(e, acc.relerr(outs[i][0].reshape(-1)[midx[int(i)]], ref))
+        return e
+
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    e_plain = errors()
+    b = BatchPlan(plans)
+    b.execute(cs, outs)  # records the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.execute(cs, outs)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    e_batch = errors()
+    evals = float(np.sum(Ms)) * ntr
+    info = b.info()
+    log(f"batched type 1: per-plan loop {t_plain * 1e3:.1f} ms, batch {t_batch * 1e3:.1f} ms, "
+        f"errors {e_plain:.2e} / {e_batch:.2e}")
+    return {
+        "workload": f"{nprob} 2D type-1 problems, N log-uniform [32,512], M log-uniform "
+                    f"[1e3,1e5], ntransf {ntr}, fp32 tol {tol:g}",
+        "speedup": t_plain / t_batch, "plain_ms": round(t_plain * 1e3, 3),
+        "batched_ms": round(t_batch * 1e3, 3), "plain_nu_evals_per_s": evals / t_plain,
+        "batched_nu_evals_per_s": evals / t_batch, "error_plain": e_plain,
+        "error_batched": e_batch, "vectors_per_round": info["vectors_per_round"],
+        "graph": info["graph"],
+    }
+
+
 def batched_comparison(nprob, ntr, tol, reps, log_every=500):
     """The uneven small-problem workload: nprob independent 2-D type-2
     problems, side N log-uniform in [32, 512], M log-uniform in [1e3, 1e5]
@@ -620,6 +700,8 @@ def main(argv=None):
                     help="time the headline problem with the default and the cost-model-"
                          "selected options on uniform and radial-spoke points (0: skip)")
     ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--batched1", type=int, default=500,
+                    help="problems of the type-1 batched comparison (0: skip)")
     ap.add_argument("--others", type=int, default=1,
                     help="also time the 2-D (uniform, clustered) and 3-D differentiation "
                          "workloads (0: skip)")
@@ -779,6 +861,11 @@ def main(argv=None):
         torch.cuda.empty_cache()
         batched = batched_comparison(args.batched, args.batched_ntr, 1e-5, args.batched_reps)
         torch.cuda.empty_cache()
+    batched1 = None
+    if args.batched1 > 0:
+        batched1 = batched_type1_comparison(args.batched1, args.batched_ntr, 1e-5,
+                                            args.batched_reps)
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -819,6 +906,7 @@ def main(argv=None):
         "c3_plain_vs_selected": c3,
         "other_configs": others,
         "batched": batched,
+        "batched_type1": batched1,
     }
     print(json.dumps(line), flush=True)
     return 0
