@@ -26,12 +26,14 @@ class FakeTensor:
         raise AssertionError("the checks must fail before any pointer is taken")
 
 
-def _bare_batch(fk_sizes, out_sizes):
-    """A BatchPlan with its per-problem sizes set and no library handle."""
+def _bare_batch(fk_sizes, out_sizes, ttype=2):
+    """A BatchPlan (of type-2 plans unless ttype says otherwise) with its
+    per-problem sizes set and no library handle."""
     b = object.__new__(BatchPlan)
     b._plans = [None] * len(fk_sizes)
     b._fk_sizes, b._out_sizes = list(fk_sizes), list(out_sizes)
     b._dtype = torch.complex64
+    b._type = ttype
     b._h = None
     return b
 
@@ -111,3 +113,17 @@ def test_autograd_two_forwards_one_backward(kind):
     assert torch.equal(got[0], x1) and torch.equal(got[1], x2)
     assert torch.allclose(d1.grad.real, torch.ones(5)) and torch.allclose(d2.grad.real,
                                                                          2 * torch.ones(5))
+
+
+def test_type1_batch_takes_strengths_in_and_modes_out():
+    """A type-1 batch's execute takes the strengths as input and the modes as
+    output: the sizes are checked that way round before the library runs."""
+    b = _bare_batch([8, 8], [5, 6], ttype=1)
+    c = [torch.empty(5, dtype=torch.complex64), torch.empty(6, dtype=torch.complex64)]
+    fk = [torch.empty(8, dtype=torch.complex64), torch.empty(8, dtype=torch.complex64)]
+    # the right sizes get as far as the device check (CPU tensors here)
+    with pytest.raises(TypeError, match="problem 0"):
+        b.execute(c, fk)
+    # swapped (type-2 order) fails the size check
+    with pytest.raises((TypeError, ValueError)):
+        b.execute(fk, c)
